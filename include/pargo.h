@@ -1,0 +1,204 @@
+/*
+ * pargo.h -- C ABI of the MI355X focal-operator library (libpargo.so).
+ *
+ * This is the device seam that PaRGO's RasterOperator<T>::Work leaves empty
+ * for CUDA_Type (gpro/rasterOperator.h:347-348).  Instead of calling a
+ * virtual Operator() per cell (rasterOperator.h:174-185), a host operator
+ * hands its layer descriptors to one entry point below, which runs the whole
+ * work rectangle on the GPU.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: PARGO_OK (0) or a negative
+ *     PARGO_E_* code.  Nothing throws and nothing calls exit().  The text of
+ *     the last failure on a context is returned by pargo_last_error(), the
+ *     counterpart of the reference's cerr messages.
+ *   - Raster and output pointers are DEVICE pointers unless a parameter says
+ *     "host".  Device memory belongs to the caller; the library never frees
+ *     caller memory.  pargo_alloc/pargo_free are conveniences.
+ *   - Calls on a context are ordered on the context's stream and return
+ *     before the GPU work finishes; pargo_sync() waits.  One context per host
+ *     thread.  The library keeps no global mutable state apart from the
+ *     per-device scratch of the no-allocation entry point, if present.
+ *   - Rasters are row-major float32 with a pitch in elements (0 means
+ *     pitch == cols).  Output rasters of a call use the same pitch as its
+ *     input.  NoData bitmasks hold ceil(cols/32) uint32 words per row, bit
+ *     (c % 32) of word (c / 32) set when cell c is NoData (including the
+ *     border rows and columns, which the reference never computes).
+ */
+#ifndef PARGO_H
+#define PARGO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PARGO_ABI_VERSION 1
+
+/* status codes */
+#define PARGO_OK 0
+#define PARGO_E_INVALID -1     /* an argument is out of range or NULL          */
+#define PARGO_E_DEVICE -2      /* a HIP runtime call failed                    */
+#define PARGO_E_NOMEM -3       /* device or host allocation failed            */
+#define PARGO_E_UNSUPPORTED -4 /* valid request this build does not implement */
+#define PARGO_E_NONTERM -5     /* the reference's loop would never terminate  */
+
+/* Slope methods, in the order of enum SlopeAlgor
+ * (apps/morphology/slope/slopeOperator.h:20-28). */
+#define PARGO_SLOPE_FD 0   /* Horn: 3rd-order FD weighted by 1/d^2 */
+#define PARGO_SLOPE_FFD 1  /* frame finite difference */
+#define PARGO_SLOPE_MD 2   /* maximum downslope */
+#define PARGO_SLOPE_SD 3   /* simple difference */
+#define PARGO_SLOPE_SFD 4  /* second-order finite difference */
+#define PARGO_SLOPE_TFD 5  /* third-order finite difference */
+#define PARGO_SLOPE_TFDW 6 /* third-order FD weighted by 1/d */
+
+/* Slope output units.  The reference writes the gradient magnitude
+ * (tan of the slope angle, slopeOperator.cpp:86). */
+#define PARGO_UNIT_TAN 0
+#define PARGO_UNIT_DEGREES 1
+
+/* D8 device codes (uint8 per cell).
+ *   dirtype 256: ESRI bit-sum of all tied steepest directions
+ *                (E=1 SE=2 S=4 SW=8 W=16 NW=32 N=64 NE=128), 0 = flat/pit
+ *                (the reference's -1), d8Operator.cpp:73-76,89-97.
+ *   dirtype 8:   median tied window index 0..8 (0=NW 1=N 2=NE 3=W 5=E
+ *                6=SW 7=S 8=SE), 255 = flat/pit (the reference's -1 read
+ *                as int8), d8Operator.cpp:78-88.
+ * NoData cells (any NoData value in the 3x3 window, and the border) are
+ * reported through the NoData bitmask or the slope output, never through
+ * the code byte, which carries no NoData value. */
+#define PARGO_D8_FLAT_256 0
+#define PARGO_D8_FLAT_8 255
+
+typedef struct pargo_ctx pargo_ctx;
+
+/* Raster descriptor: the fields of gpro's MetaData that the focal kernels
+ * use (gpro/metaData.h:55-58).  cellsize is rounded to float inside the
+ * library, because MetaData::cellSize is a float (metaData.h:56). */
+typedef struct {
+    int32_t rows;
+    int32_t cols;
+    int64_t pitch;   /* elements between row starts; 0 = cols */
+    double cellsize;
+    double nodata;
+} pargo_grid;
+
+/* ---- context ------------------------------------------------------- */
+
+/* Opens a context on HIP device `device` with its own stream.  Replaces
+ * Application::START's program-type switch (gpro/application.cpp:5-18). */
+int pargo_create(int device, pargo_ctx **out);
+int pargo_destroy(pargo_ctx *ctx);
+/* Message of the last failure on ctx; with ctx == NULL, of the last failure
+ * on this thread that had no context (e.g. pargo_create). */
+const char *pargo_last_error(const pargo_ctx *ctx);
+int pargo_abi_version(void);
+/* Runs subsequent work on `hip_stream` (a hipStream_t; NULL is HIP's
+ * default stream) instead of the context's own stream.
+ * pargo_set_stream(ctx, pargo_own_stream(ctx)) switches back. */
+int pargo_set_stream(pargo_ctx *ctx, void *hip_stream);
+void *pargo_own_stream(const pargo_ctx *ctx);
+int pargo_sync(pargo_ctx *ctx);
+
+int pargo_alloc(pargo_ctx *ctx, size_t bytes, void **dptr);
+int pargo_free(pargo_ctx *ctx, void *dptr);
+int pargo_h2d(pargo_ctx *ctx, void *dst, const void *host_src, size_t bytes);
+int pargo_d2h(pargo_ctx *ctx, void *host_dst, const void *src, size_t bytes);
+
+/* ---- focal operators (3x3 Moore neighbourhood) --------------------- */
+
+/* Slope of every interior cell; border cells and cells with NoData in their
+ * window get grid->nodata.  Replaces SlopeOperator::Operator
+ * (apps/morphology/slope/slopeOperator.cpp:51-139) driven by
+ * RasterOperator::Work (gpro/rasterOperator.h:171-185).
+ * nodata_mask_out may be NULL. */
+int pargo_slope(pargo_ctx *ctx, const pargo_grid *grid, const float *dem,
+                int method, int unit, float *slope_out, uint32_t *nodata_mask_out);
+
+/* D8 flow direction codes (see PARGO_D8_*).  Replaces D8Operator::Operator
+ * (apps/hydrology/flowdird8/d8Operator.cpp:28-102); dirtype as in
+ * D8Operator(int) (d8Operator.h:16-21), 8 or 256.  Decisions are made on
+ * the same FP64 values as the reference, so codes are bit-exact. */
+int pargo_d8(pargo_ctx *ctx, const pargo_grid *grid, const float *dem, int dirtype,
+             uint8_t *code_out, uint32_t *nodata_mask_out);
+
+/* Slope and D8 in one pass over the DEM (the reference runs them as two
+ * programs, pSlope.cpp and pD8.cpp).  Either output may be NULL. */
+int pargo_slope_d8(pargo_ctx *ctx, const pargo_grid *grid, const float *dem,
+                   int method, int unit, int dirtype, float *slope_out,
+                   uint8_t *code_out, uint32_t *nodata_mask_out);
+
+/* ---- IDW ------------------------------------------------------------ */
+
+/* Output grid of IDW for these samples (host arrays, file order), by the
+ * extent rule of IDWOperator::readSamples (idwOperator.cpp:126-141). */
+int pargo_idw_grid(const double *xs, const double *ys, int n, float cellsize,
+                   int *rows, int *cols, double *min_x, double *max_y);
+
+typedef struct pargo_idw_plan pargo_idw_plan;
+
+/* Buckets the samples (host arrays, file order) into blockSize blocks like
+ * IDWOperator::creatSampleBlocks (idwOperator.cpp:152-169) and uploads them.
+ * The plan belongs to the context that made it. */
+int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const double *vals,
+                      int n, float cellsize, double block_size, pargo_idw_plan **out);
+int pargo_idw_plan_free(pargo_ctx *ctx, pargo_idw_plan *plan);
+/* Replaces new sample values (host array, file order) without re-bucketing. */
+int pargo_idw_set_values(pargo_ctx *ctx, pargo_idw_plan *plan, const double *vals);
+
+/* IDW of every output cell: IDWOperator::Operator + searchNbrSamples
+ * (idwOperator.cpp:332-491) with 1-rank semantics.  out is a device float
+ * grid of rows x cols from pargo_idw_grid.  mask (device int32,
+ * mask_rows x mask_cols, may be NULL) is looked up like
+ * rowAtOtherLayer/colAtOtherLayer (gpro/rasterLayer.h:1296-1316); masked
+ * cells get -9999 (idwOperator.h:14).  Returns PARGO_E_NONTERM when
+ * buffer <= 0 and n < K, where the reference loops forever.
+ * candidates_out (device uint64, may be NULL) receives the number of
+ * (cell, sample) distance evaluations. */
+int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                  const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
+                  float *out, unsigned long long *candidates_out);
+
+/* ---- adjoints (no counterpart in the reference) ---------------------- */
+
+/* dL/dz of the Horn slope (PARGO_SLOPE_FD, PARGO_UNIT_TAN) given dL/dslope
+ * per cell.  Cells whose slope is NoData or a border cell take no gradient;
+ * a cell with zero gradient magnitude contributes 0.  Only the DEM is read:
+ * the stencil intermediates are recomputed. */
+int pargo_slope_adjoint(pargo_ctx *ctx, const pargo_grid *grid, const float *dem,
+                        const float *grad_slope, float *grad_dem);
+
+/* dL/d(sample value) of IDW given dL/d(out) per cell.  grad_vals is a
+ * device double array of n entries in file order, overwritten. */
+int pargo_idw_adjoint(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                      const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
+                      const float *grad_out, double *grad_vals);
+
+/* ---- synthetic inputs (deterministic; SplitMix64 seeded) ------------ */
+
+/* Fractal value-noise DEM on the host: 8 octaves, persistence 0.5, base
+ * wavelength max(rows,cols)/4 cells, normalised to [0, 2000] m, float32. */
+int pargo_synth_dem(uint64_t seed, int rows, int cols, float *host_out, int nthreads);
+/* Rectangular plateaus at constant elevation until `fraction` of the cells
+ * are covered (sides 3..50 cells). */
+int pargo_synth_plateaus(uint64_t seed, int rows, int cols, float *host_dem, double fraction);
+/* NoData discs (radius 5..40 cells): n_discs > 0 places that many discs,
+ * otherwise discs are added until `fraction` of the cells are covered.
+ * Returns the number of NoData cells through covered (may be NULL). */
+int pargo_synth_nodata_discs(uint64_t seed, int rows, int cols, float *host_dem, int n_discs,
+                             double fraction, float nodata, int64_t *covered);
+/* IDW samples: n/2 uniform over the extent and n - n/2 from a 10-component
+ * Gaussian mixture (sigma 1-5 km), values from the same noise field plus
+ * N(0, 1 m), all float32-representable; the first two samples are the
+ * anchors (x0+cs/2, y0+cs/2) and (x0+W-cs/2, y0+H-cs/2). */
+int pargo_synth_samples(uint64_t seed, int n, double x0, double y0, double width, double height,
+                        double cellsize, double *xs, double *ys, double *vals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARGO_H */

@@ -1,0 +1,173 @@
+"""ctypes binding of libpargo.so (the C ABI declared in include/pargo.h).
+
+The library is loaded from pargo_amd/_build/libpargo.so.  There is no
+fallback: if the library is missing, or if a GPU entry point is called on a
+machine without a HIP device, the call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libpargo.so")
+
+PARGO_OK = 0
+PARGO_E_INVALID = -1
+PARGO_E_DEVICE = -2
+PARGO_E_NOMEM = -3
+PARGO_E_UNSUPPORTED = -4
+PARGO_E_NONTERM = -5
+
+SLOPE_METHODS = {"FD": 0, "FFD": 1, "MD": 2, "SD": 3, "SFD": 4, "TFD": 5, "TFDW": 6}
+UNIT_TAN = 0
+UNIT_DEGREES = 1
+D8_FLAT_256 = 0
+D8_FLAT_8 = 255
+
+# Every exported symbol of include/pargo.h with (restype, argtypes).
+_c = ctypes
+_P = _c.c_void_p
+_SIGS = {
+    "pargo_abi_version": (_c.c_int, []),
+    "pargo_create": (_c.c_int, [_c.c_int, _c.POINTER(_P)]),
+    "pargo_destroy": (_c.c_int, [_P]),
+    "pargo_last_error": (_c.c_char_p, [_P]),
+    "pargo_set_stream": (_c.c_int, [_P, _P]),
+    "pargo_own_stream": (_P, [_P]),
+    "pargo_sync": (_c.c_int, [_P]),
+    "pargo_alloc": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_P)]),
+    "pargo_free": (_c.c_int, [_P, _P]),
+    "pargo_h2d": (_c.c_int, [_P, _P, _P, _c.c_size_t]),
+    "pargo_d2h": (_c.c_int, [_P, _P, _P, _c.c_size_t]),
+    "pargo_slope": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _P, _P]),
+    "pargo_d8": (_c.c_int, [_P, _P, _P, _c.c_int, _P, _P]),
+    "pargo_slope_d8": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _P, _P, _P]),
+    "pargo_idw_grid": (_c.c_int, [_P, _P, _c.c_int, _c.c_float, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int),
+                                  _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "pargo_idw_prepare": (_c.c_int, [_P, _P, _P, _P, _c.c_int, _c.c_float, _c.c_double, _c.POINTER(_P)]),
+    "pargo_idw_plan_free": (_c.c_int, [_P, _P]),
+    "pargo_idw_set_values": (_c.c_int, [_P, _P, _P]),
+    "pargo_idw_run": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int, _c.c_int,
+                                 _P, _P]),
+    "pargo_slope_adjoint": (_c.c_int, [_P, _P, _P, _P, _P]),
+    "pargo_idw_adjoint": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
+                                     _c.c_int, _P, _P]),
+    "pargo_synth_dem": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_int]),
+    "pargo_synth_plateaus": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_double]),
+    "pargo_synth_nodata_discs": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_double,
+                                            _c.c_float, _c.POINTER(_c.c_int64)]),
+    "pargo_synth_samples": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_double, _c.c_double, _c.c_double,
+                                       _c.c_double, _c.c_double, _P, _P, _P]),
+}
+
+
+class PargoGrid(ctypes.Structure):
+    """pargo_grid (include/pargo.h)."""
+    _fields_ = [("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("pitch", ctypes.c_int64),
+                ("cellsize", ctypes.c_double), ("nodata", ctypes.c_double)]
+
+
+class PargoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pargo error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libpargo.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libpargo.so not found at %s; run python -m pargo_amd.build" % LIB_PATH)
+        # torch ships its own HIP runtime (soname libamdhip64.so.7).  Load it
+        # first so that libpargo binds to that same runtime by soname: two
+        # HIP runtimes in one process do not share the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue  # reported by missing_symbols()
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def missing_symbols():
+    """Entry points of include/pargo.h that the loaded library lacks."""
+    L = lib()
+    return [n for n in _SIGS if getattr(L, n, None) is None]
+
+
+def _ptr(t):
+    """Device or host pointer of a torch tensor / numpy array / None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    if hasattr(t, "ctypes"):
+        return ctypes.c_void_p(t.ctypes.data)
+    return ctypes.c_void_p(int(t))
+
+
+class Context:
+    """A pargo_ctx bound to one device.  By default work is queued on torch's
+    current stream of that device, so results are ordered with torch ops."""
+
+    def __init__(self, device=0, use_torch_stream=True):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.pargo_create(int(device), ctypes.byref(h))
+        if rc != PARGO_OK:
+            msg = L.pargo_last_error(None)
+            raise PargoError(rc, "pargo_create(%d): %s" % (device, msg.decode() if msg else ""))
+        self.h = h
+        self.device = int(device)
+        self.use_torch_stream = use_torch_stream
+
+    def close(self):
+        if self.h:
+            lib().pargo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != PARGO_OK:
+            msg = lib().pargo_last_error(self.h)
+            raise PargoError(rc, msg.decode() if msg else "")
+        return rc
+
+    def bind_stream(self):
+        if self.use_torch_stream:
+            import torch
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            self.check(lib().pargo_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def sync(self):
+        self.check(lib().pargo_sync(self.h))
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    ctx = _default_ctx.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _default_ctx[device] = ctx
+    return ctx

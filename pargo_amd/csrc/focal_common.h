@@ -1,0 +1,42 @@
+// focal_common.h -- device helpers shared by the focal (3x3) kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pargo {
+
+// sqrt(2.0) as the reference evaluates it (glibc sqrt is correctly rounded)
+// and the double nearest to its reciprocal.
+constexpr double kSqrt2 = 0x1.6a09e667f3bcdp+0;
+constexpr double kInvSqrt2 = 0x1.6a09e667f3bccp-1;
+// The D8 epsilon 0.000001 (apps/hydrology/flowdird8/d8Operator.cpp:3).
+constexpr double kD8Eps = 0.000001;
+
+// RN(x / kSqrt2) for finite x derived from float32 data (|x| <= 2^129,
+// x = 0 or |x| >= 2^-149, so nothing below underflows).
+// q0 = RN(x*y) with y = RN(1/s) is within 0.69 ulp of x/s because
+// |y*s - 1| = 0.183 * 2^-53; by Markstein's theorem (y within half an ulp
+// of 1/s, q0 within one ulp of x/s) r = x - q0*s is exact in one FMA and
+// RN(q0 + r*y) is the correctly rounded quotient.  The identity is also
+// checked on the GPU against IEEE division by the test suite.
+__device__ __forceinline__ double div_sqrt2(double x)
+{
+    double q0 = x * kInvSqrt2;
+    double r = __builtin_fma(-q0, kSqrt2, x);
+    return __builtin_fma(r, kInvSqrt2, q0);
+}
+
+// Exact float interval test for NoData (see nodata_interval on the host).
+__device__ __forceinline__ bool in_interval(float v, float lo, float hi) { return v >= lo && v <= hi; }
+
+// ESRI code -> reference dirType 8 output (median tied window index).
+// Built on the host from the reference's rule validDir[len >> 1]
+// (d8Operator.cpp:78-88); 255 for code 0 (unused: flat cells are
+// resolved before the lookup).
+struct D8Lut8 {
+    uint8_t v[256];
+};
+
+}  // namespace pargo

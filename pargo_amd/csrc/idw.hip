@@ -1,0 +1,453 @@
+// idw.hip -- inverse-distance-weighted interpolation with the reference's
+// block ring search (1-rank semantics).
+//
+// Reference behaviour reproduced here:
+//   extent          apps/spatial/idw/idwOperator.cpp:126-141 (float cellSize)
+//   sample blocks   idwOperator.cpp:152-169, idwOperator.h:68-69 (file order)
+//   cell centres    idwOperator.h:91-92
+//   ring search     idwOperator.cpp:332-437 (buffer stop before a ring,
+//                   strict-< replacement of the first maximum slot, stop
+//                   once K are found and (r-1)*B + m >= maxDist)
+//   block bound m   idwOperator.cpp:5-20 with the global extent (one rank)
+//   weights         idwOperator.cpp:470-480, zero distance -> 1e-7
+//   mask            idwOperator.cpp:446-461, gpro/rasterLayer.h:1296-1316
+//
+// Device layout: samples bucketed by block into CSR order (file order inside
+// a block), x/y interleaved as double2 so one candidate is one 16-byte load,
+// values and the file index of each bucket slot in separate arrays.
+//
+// Work split: one lane per output cell, a wave covers an 8x8 cell tile, so
+// its 64 lanes almost always share one sample block and walk the same ring
+// and block sequence (the loads of a candidate hit one address).  Lanes keep
+// their K nearest (distance, bucket slot) pairs in registers.
+//
+// Exactness: squared distances are formed as RN(RN(dx*dx) + RN(dy*dy)) from
+// dx = RN(xs - cx), the operands of the reference's sqrt(pow(dx,2) +
+// pow(dy,2)).  A candidate is pre-filtered by s <= maxDist^2 * (1 + 2^-40),
+// a necessary condition for tmpDist < maxDist; survivors get the correctly
+// rounded FP64 sqrt and the reference's exact comparisons, so the K-set and
+// its slot order equal the reference's.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+struct pargo_idw_plan {
+    pargo_ctx *owner = nullptr;
+    int n = 0;
+    float cs = 0;
+    double block = 0;
+    double minX = 0, maxX = 0, minY = 0, maxY = 0;
+    int rows = 0, cols = 0;
+    int blockRows = 0, blockCols = 0;
+    void *d_mem = nullptr;
+    double2 *d_xy = nullptr;   // bucket order
+    double *d_v = nullptr;     // bucket order
+    int *d_start = nullptr;    // blockRows*blockCols + 1
+    int *d_perm = nullptr;     // bucket slot -> file index
+    std::vector<int> h_perm;
+};
+
+namespace pargo {
+
+// readSamples extent rule.  "_cellSize / 2" and "totalcol * _cellSize" are
+// float expressions in the reference and are kept float here.
+static void idw_extent(const double *xs, const double *ys, int n, float cs, double &minX, double &maxX,
+                       double &minY, double &maxY, int &rows, int &cols)
+{
+    double mnx = xs[0], mxx = xs[0], mny = ys[0], mxy = ys[0];
+    for (int i = 1; i < n; i++) {
+        if (xs[i] > mxx) mxx = xs[i];
+        if (xs[i] < mnx) mnx = xs[i];
+        if (ys[i] > mxy) mxy = ys[i];
+        if (ys[i] < mny) mny = ys[i];
+    }
+    mnx = mnx - (double)(cs / 2);
+    int tc = (int)((mxx - mnx) / (double)cs);
+    if ((mxx - mnx) != (double)((float)tc * cs)) tc++;
+    mxx = mnx + (double)(cs * (float)tc);
+    mny = mny - (double)(cs / 2);
+    int tr = (int)((mxy - mny) / (double)cs);
+    if ((mxy - mny) != (double)((float)tr * cs)) tr++;
+    mxy = mny + (double)(cs * (float)tr);
+    minX = mnx; maxX = mxx; minY = mny; maxY = mxy;
+    rows = tr; cols = tc;
+}
+
+struct IdwParams {
+    const double2 *__restrict__ xy;
+    const double *__restrict__ v;
+    const int *__restrict__ start;
+    const int32_t *__restrict__ mask;
+    float *__restrict__ out;
+    unsigned long long *__restrict__ cand_out;
+    double grad_scale_unused;
+    double minX, maxY, block, buffer, cs;
+    int rows, cols, blockRows, blockCols, maxRing;
+    int k, power;
+    int mask_rows, mask_cols, mask_nodata;
+};
+
+constexpr double kIdwEps = 0.0000001;  // idwOperator.h:13
+constexpr double kPrefilterSlack = 1.0 + 0x1p-40;
+
+// rowAtOtherLayer: (int)min(scale * idx, other - 1) with double scale.
+__device__ __forceinline__ int mask_index(int other, int self, int idx)
+{
+    double scale = (double)other / (double)self;
+    double v = scale * (double)idx;
+    double lim = (double)(other - 1);
+    return (int)((lim < v) ? lim : v);
+}
+
+__device__ __forceinline__ double powi(double d, int p)
+{
+    // pow(d, p) for integer p; the result only feeds weights (1e-5 bar).
+    if (p == 2) return d * d;
+    if (p == 1) return d;
+    double r = 1.0, b = d;
+    unsigned e = p < 0 ? (unsigned)(-p) : (unsigned)p;
+    while (e) {
+        if (e & 1) r *= b;
+        b *= b;
+        e >>= 1;
+    }
+    return p < 0 ? 1.0 / r : r;
+}
+
+// The K-nearest search of one cell.  Fills dist[]/slot[] like nbrSamples
+// and returns min(found, K).  Counts distance evaluations in *ncand.
+template <int MAXK>
+__device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, double cellY, double dist[MAXK],
+                                          int slot[MAXK], unsigned &ncand)
+{
+    const int K = p.k;
+    const int blockRow = (int)((p.maxY - cellY) / p.block);
+    const int blockCol = (int)((cellX - p.minX) / p.block);
+    // getMinDistanceToBlockBound (idwOperator.cpp:5-20), one rank
+    const double bx0 = blockCol * p.block + p.minX, bx1 = (blockCol + 1) * p.block + p.minX;
+    const double by0 = p.maxY - blockRow * p.block, by1 = p.maxY - (blockRow + 1) * p.block;
+    const double d0 = fabs(cellX - bx0), d1 = fabs(cellX - bx1), d2 = fabs(cellY - by0), d3 = fabs(cellY - by1);
+    const double a = (d1 < d0) ? d1 : d0, b = (d3 < d2) ? d3 : d2;
+    const double minDistToBound = (b < a) ? b : a;
+
+    double maxDist = 0.0;
+    int maxIdx = -1;
+    int tail = -1;
+    double kthr = INFINITY;  // prefilter on squared distance while filling
+#pragma unroll
+    for (int i = 0; i < MAXK; i++) {
+        dist[i] = 0.0;
+        slot[i] = 0;
+    }
+    for (int rad = 0;; rad++) {
+        const double searchRange = (double)(rad - 1) * p.block + minDistToBound;
+        if (p.buffer > 0 && searchRange >= p.buffer) break;
+        if (rad > p.maxRing) break;  // every further ring is outside the block grid
+        for (int tRow = blockRow - rad; tRow <= blockRow + rad; ++tRow) {
+            if (tRow < 0 || tRow >= p.blockRows) continue;
+            const bool full = (tRow == blockRow - rad) || (tRow == blockRow + rad);
+            // full top/bottom rows, only the two end columns in between
+            const int step = full ? 1 : 2 * rad;
+            for (int tCol = blockCol - rad; tCol <= blockCol + rad; tCol += step) {
+                if (tCol < 0 || tCol >= p.blockCols) continue;
+                const int bi = tRow * p.blockCols + tCol;
+                const int s0 = p.start[bi], s1 = p.start[bi + 1];
+                ncand += (unsigned)(s1 - s0);
+                for (int q = s0; q < s1; q++) {
+                    const double2 xy = p.xy[q];
+                    const double dx = xy.x - cellX, dy = xy.y - cellY;
+                    const double s = dx * dx + dy * dy;
+                    if (s <= kthr) {
+                        const double tmpDist = (s == 0.0) ? kIdwEps : sqrt(s);
+                        if (tail < K - 1) {
+                            tail++;
+#pragma unroll
+                            for (int i = 0; i < MAXK; i++)
+                                if (i == tail) {
+                                    dist[i] = tmpDist;
+                                    slot[i] = q;
+                                }
+                            if (tmpDist > maxDist) {
+                                maxDist = tmpDist;
+                                maxIdx = tail;
+                            }
+                            if (tail == K - 1) kthr = maxDist * maxDist * kPrefilterSlack;
+                        } else {
+                            // (the reference also counts tailIdx up here;
+                            // past K-1 only "tail >= K-1" is ever used)
+                            if (tmpDist < maxDist) {
+#pragma unroll
+                                for (int i = 0; i < MAXK; i++)
+                                    if (i == maxIdx) {
+                                        dist[i] = tmpDist;
+                                        slot[i] = q;
+                                    }
+                                maxDist = dist[0];
+                                maxIdx = 0;
+#pragma unroll
+                                for (int i = 1; i < MAXK; i++)
+                                    if (i < K && dist[i] > maxDist) {
+                                        maxDist = dist[i];
+                                        maxIdx = i;
+                                    }
+                                kthr = maxDist * maxDist * kPrefilterSlack;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (tail >= K - 1 && searchRange >= maxDist) break;
+    }
+    return (tail + 1 < K) ? tail + 1 : K;
+}
+
+template <int MAXK>
+__global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int col = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    unsigned ncand = 0;
+    if (row < p.rows && col < p.cols) {
+        float result;
+        bool skip = false;
+        if (p.mask) {
+            const int mr = mask_index(p.mask_rows, p.rows, row);
+            const int mc = mask_index(p.mask_cols, p.cols, col);
+            const int m = p.mask[(long long)mr * p.mask_cols + mc];
+            skip = (m == 0) | (m == p.mask_nodata);
+        }
+        if (skip) {
+            result = -9999.0f;
+        } else {
+            const double cellX = (double)((col + 0.5) * (double)p.cs + p.minX);
+            const double cellY = (double)(p.maxY - (row + 0.5) * (double)p.cs);
+            double dist[MAXK];
+            int slot[MAXK];
+            const int num = idw_search<MAXK>(p, cellX, cellY, dist, slot, ncand);
+            double wsum = 0.0, w[MAXK];
+#pragma unroll
+            for (int i = 0; i < MAXK; i++) {
+                w[i] = 0.0;
+                if (i < num) {
+                    w[i] = 1.0 / powi(dist[i], p.power);
+                    wsum += w[i];
+                }
+            }
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < MAXK; i++)
+                if (i < num) acc += p.v[slot[i]] * w[i] / wsum;
+            result = (float)acc;
+        }
+        p.out[(long long)row * p.cols + col] = result;
+    }
+    if (p.cand_out) {
+        unsigned long long c = ncand;
+        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0 && c) atomicAdd(p.cand_out, c);
+    }
+}
+
+template <int MAXK>
+static void launch_idw_k(const IdwParams &p, hipStream_t s)
+{
+    dim3 block(256);
+    dim3 grid((p.cols + 15) / 16, (p.rows + 15) / 16);
+    hipLaunchKernelGGL((idw_kernel<MAXK>), grid, block, 0, s, p);
+}
+
+static int launch_idw(const IdwParams &p, hipStream_t s)
+{
+    if (p.k <= 4) launch_idw_k<4>(p, s);
+    else if (p.k <= 8) launch_idw_k<8>(p, s);
+    else if (p.k <= 12) launch_idw_k<12>(p, s);
+    else if (p.k <= 16) launch_idw_k<16>(p, s);
+    else if (p.k <= 32) launch_idw_k<32>(p, s);
+    else return PARGO_E_UNSUPPORTED;
+    return PARGO_OK;
+}
+
+int idw_fill_params(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                    const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, IdwParams &p)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!plan || plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "IDW plan is null or from another context");
+    if (k < 1) return fail(ctx, PARGO_E_INVALID, "searchPointNum must be >= 1, got %d", k);
+    if (k > 32) return fail(ctx, PARGO_E_UNSUPPORTED, "searchPointNum %d > 32 is not supported", k);
+    if (!(buffer > 0) && plan->n < k)
+        return fail(ctx, PARGO_E_NONTERM,
+                    "searchRange <= 0 with %d samples < searchPointNum %d: the reference's search never ends",
+                    plan->n, k);
+    if (mask && (mask_rows < 1 || mask_cols < 1)) return fail(ctx, PARGO_E_INVALID, "bad mask dimensions");
+    p = IdwParams{};
+    p.xy = plan->d_xy;
+    p.v = plan->d_v;
+    p.start = plan->d_start;
+    p.mask = mask;
+    p.minX = plan->minX;
+    p.maxY = plan->maxY;
+    p.block = plan->block;
+    p.buffer = buffer;
+    p.cs = (double)plan->cs;
+    p.rows = plan->rows;
+    p.cols = plan->cols;
+    p.blockRows = plan->blockRows;
+    p.blockCols = plan->blockCols;
+    p.maxRing = std::max(plan->blockRows, plan->blockCols);
+    p.k = k;
+    p.power = power;
+    p.mask_rows = mask_rows;
+    p.mask_cols = mask_cols;
+    p.mask_nodata = mask_nodata;
+    return PARGO_OK;
+}
+
+}  // namespace pargo
+
+using pargo::fail;
+
+extern "C" {
+
+int pargo_idw_grid(const double *xs, const double *ys, int n, float cellsize, int *rows, int *cols, double *min_x,
+                   double *max_y)
+{
+    if (!xs || !ys || n < 1 || !(cellsize > 0) || !rows || !cols) return PARGO_E_INVALID;
+    double a, b, c, d;
+    pargo::idw_extent(xs, ys, n, cellsize, a, b, c, d, *rows, *cols);
+    if (min_x) *min_x = a;
+    if (max_y) *max_y = d;
+    return PARGO_OK;
+}
+
+int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const double *vals, int n,
+                      float cellsize, double block_size, pargo_idw_plan **out)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!out || !xs || !ys || !vals || n < 1) return fail(ctx, PARGO_E_INVALID, "pargo_idw_prepare: bad arguments");
+    if (!(cellsize > 0) || !std::isfinite(cellsize)) return fail(ctx, PARGO_E_INVALID, "resolution must be > 0");
+    if (!(block_size > 0) || !std::isfinite(block_size)) return fail(ctx, PARGO_E_INVALID, "blockSize must be > 0");
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(xs[i]) || !std::isfinite(ys[i]))
+            return fail(ctx, PARGO_E_INVALID, "sample %d has a non-finite coordinate", i);
+    *out = nullptr;
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    pargo_idw_plan *pl = new (std::nothrow) pargo_idw_plan();
+    if (!pl) return fail(ctx, PARGO_E_NOMEM, "out of host memory");
+    pl->owner = ctx;
+    pl->n = n;
+    pl->cs = cellsize;
+    pl->block = block_size;
+    pargo::idw_extent(xs, ys, n, cellsize, pl->minX, pl->maxX, pl->minY, pl->maxY, pl->rows, pl->cols);
+    if (pl->rows < 1 || pl->cols < 1) {
+        delete pl;
+        return fail(ctx, PARGO_E_INVALID, "empty output grid");
+    }
+    // creatSampleBlocks: ceil of the extent over the block size
+    const double br = std::ceil((pl->maxY - pl->minY) / block_size), bc = std::ceil((pl->maxX - pl->minX) / block_size);
+    if (br * bc > 1 << 26) {
+        delete pl;
+        return fail(ctx, PARGO_E_INVALID, "too many sample blocks (%.0f x %.0f)", br, bc);
+    }
+    pl->blockRows = (int)br;
+    pl->blockCols = (int)bc;
+    const int nb = pl->blockRows * pl->blockCols;
+    std::vector<int> start(nb + 1, 0), bidx(n);
+    for (int i = 0; i < n; i++) {
+        const int iCol = (int)((xs[i] - pl->minX) / block_size);
+        const int iRow = (int)((pl->maxY - ys[i]) / block_size);
+        const long long index = (long long)iRow * pl->blockCols + iCol;
+        if (iCol < 0 || iRow < 0 || iCol >= pl->blockCols || index >= nb) {
+            delete pl;
+            return fail(ctx, PARGO_E_INVALID, "Index out of range. An error in creatSampleBlocks() (sample %d)", i);
+        }
+        bidx[i] = (int)index;
+        start[index + 1]++;
+    }
+    for (int b = 0; b < nb; b++) start[b + 1] += start[b];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    std::vector<double2> xy(n);
+    std::vector<double> v(n);
+    pl->h_perm.resize(n);
+    for (int i = 0; i < n; i++) {
+        const int q = fill[bidx[i]]++;
+        xy[q] = make_double2(xs[i], ys[i]);
+        v[q] = vals[i];
+        pl->h_perm[q] = i;
+    }
+    const size_t bytes_xy = sizeof(double2) * n, bytes_v = sizeof(double) * n;
+    const size_t bytes_start = sizeof(int) * (nb + 1), bytes_perm = sizeof(int) * n;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = al(bytes_xy) + al(bytes_v) + al(bytes_start) + al(bytes_perm);
+    hipError_t e = hipMalloc(&pl->d_mem, total);
+    if (e != hipSuccess) {
+        delete pl;
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for IDW samples: %s", total, hipGetErrorString(e));
+    }
+    char *base = (char *)pl->d_mem;
+    pl->d_xy = (double2 *)base;
+    pl->d_v = (double *)(base + al(bytes_xy));
+    pl->d_start = (int *)(base + al(bytes_xy) + al(bytes_v));
+    pl->d_perm = (int *)(base + al(bytes_xy) + al(bytes_v) + al(bytes_start));
+    e = hipMemcpyAsync(pl->d_xy, xy.data(), bytes_xy, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->d_v, v.data(), bytes_v, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->d_start, start.data(), bytes_start, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->d_perm, pl->h_perm.data(), bytes_perm, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(pl->d_mem);
+        delete pl;
+        return fail(ctx, PARGO_E_DEVICE, "uploading IDW samples: %s", hipGetErrorString(e));
+    }
+    *out = pl;
+    return PARGO_OK;
+}
+
+int pargo_idw_plan_free(pargo_ctx *ctx, pargo_idw_plan *plan)
+{
+    if (!ctx || !plan) return PARGO_E_INVALID;
+    if (plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "IDW plan belongs to another context");
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    hipStreamSynchronize(ctx->stream);
+    if (plan->d_mem) hipFree(plan->d_mem);
+    delete plan;
+    return PARGO_OK;
+}
+
+int pargo_idw_set_values(pargo_ctx *ctx, pargo_idw_plan *plan, const double *vals)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!plan || !vals || plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "pargo_idw_set_values: bad arguments");
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    std::vector<double> v(plan->n);
+    for (int q = 0; q < plan->n; q++) v[q] = vals[plan->h_perm[q]];
+    PARGO_HIP(ctx, hipMemcpyAsync(plan->d_v, v.data(), sizeof(double) * plan->n, hipMemcpyHostToDevice, ctx->stream));
+    PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PARGO_OK;
+}
+
+int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                  const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, float *out,
+                  unsigned long long *candidates_out)
+{
+    pargo::IdwParams p;
+    int rc = pargo::idw_fill_params(ctx, plan, k, power, buffer, mask, mask_rows, mask_cols, mask_nodata, p);
+    if (rc) return rc;
+    if (!out) return fail(ctx, PARGO_E_INVALID, "pargo_idw_run: null output");
+    rc = pargo::bind(ctx);
+    if (rc) return rc;
+    p.out = out;
+    p.cand_out = candidates_out;
+    if (candidates_out) PARGO_HIP(ctx, hipMemsetAsync(candidates_out, 0, sizeof(unsigned long long), ctx->stream));
+    rc = pargo::launch_idw(p, ctx->stream);
+    if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
+    return pargo::check_launch(ctx, "pargo_idw_run");
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// internal.h -- shared definitions of libpargo (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/pargo.h"
+
+struct pargo_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;  // where work goes (own or caller's)
+    std::string last_error;
+    int num_cu = 0;
+    // Small reusable device scratch owned by the context (grows on demand).
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+namespace pargo {
+
+// Records a formatted message on the context and returns `code`.
+int fail(pargo_ctx *ctx, int code, const char *fmt, ...);
+
+// Makes ctx->device current for the calling thread.
+int bind(pargo_ctx *ctx);
+
+// Checks the HIP error state after a launch.
+int check_launch(pargo_ctx *ctx, const char *what);
+
+// Grows the context scratch to at least `bytes`.
+int ensure_scratch(pargo_ctx *ctx, size_t bytes);
+
+// Float interval [lo, hi] of the float32 values v for which the reference's
+// double test fabs((double)v - nodata) < eps holds (empty when lo > hi).
+void nodata_interval(double nodata, double eps, float *lo, float *hi);
+
+}  // namespace pargo
+
+#define PARGO_HIP(ctx, call)                                                            \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return pargo::fail((ctx), PARGO_E_DEVICE, "%s failed: %s", #call,           \
+                               hipGetErrorString(_e));                                  \
+    } while (0)

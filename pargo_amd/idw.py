@@ -1,0 +1,105 @@
+"""Python mirror of IDWOperator over the C ABI.
+
+Reference interface: IDWOperator(float cellsize, int nbrPoints, int idw_power,
+double bufferSize, double blockSize) with readSamples / creatSampleBlocks /
+idwLayer / maskLayer / Run (apps/spatial/idw/idwOperator.h:35-116).  Here
+the samples come as arrays in file order, creatSampleBlocks is
+pargo_idw_prepare, and run() fills a device float32 grid.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def grid(xs, ys, cellsize):
+    """(rows, cols, minX, maxY) of the output grid (idwOperator.cpp:126-141)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    r, c = ctypes.c_int(), ctypes.c_int()
+    mx, my = ctypes.c_double(), ctypes.c_double()
+    rc = _lib.lib().pargo_idw_grid(_lib._ptr(xs), _lib._ptr(ys), len(xs), float(cellsize), ctypes.byref(r),
+                                   ctypes.byref(c), ctypes.byref(mx), ctypes.byref(my))
+    if rc != 0:
+        raise _lib.PargoError(rc, "pargo_idw_grid: bad arguments")
+    return r.value, c.value, mx.value, my.value
+
+
+class IDWOperator:
+    def __init__(self, cellsize, nbr_points, idw_power, buffer_size, block_size, ctx=None, device=0):
+        self.cellsize = float(np.float32(cellsize))
+        self.k = int(nbr_points)
+        self.power = int(idw_power)
+        self.buffer = float(buffer_size)
+        self.block = float(block_size)
+        self.ctx = ctx or _lib.default_context(device)
+        self.plan = None
+        self.n = 0
+        self.mask = None
+        self.mask_nodata = 0
+
+    def read_samples(self, xs, ys, vals):
+        """readSamples + creatSampleBlocks (idwOperator.cpp:63-169)."""
+        self.free()
+        xs = np.ascontiguousarray(xs, dtype=np.float64)
+        ys = np.ascontiguousarray(ys, dtype=np.float64)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self.n = len(xs)
+        self.rows, self.cols, self.min_x, self.max_y = grid(xs, ys, self.cellsize)
+        h = ctypes.c_void_p()
+        self.ctx.bind_stream()
+        self.ctx.check(_lib.lib().pargo_idw_prepare(self.ctx.h, _lib._ptr(xs), _lib._ptr(ys), _lib._ptr(vals),
+                                                    self.n, self.cellsize, self.block, ctypes.byref(h)))
+        self.plan = h
+        return self
+
+    def set_values(self, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self.ctx.check(_lib.lib().pargo_idw_set_values(self.ctx.h, self.plan, _lib._ptr(vals)))
+
+    def mask_layer(self, mask, mask_nodata):
+        """maskLayer: a device int32 grid; 0 and mask_nodata cells give -9999."""
+        self.mask = mask
+        self.mask_nodata = int(mask_nodata)
+
+    def run(self, out=None, count_candidates=False):
+        import torch
+        dev = torch.device("cuda", self.ctx.device)
+        if out is None:
+            out = torch.empty((self.rows, self.cols), dtype=torch.float32, device=dev)
+        cand = torch.zeros(1, dtype=torch.int64, device=dev) if count_candidates else None
+        mr = mc = 0
+        if self.mask is not None:
+            mr, mc = self.mask.shape
+        self.ctx.bind_stream()
+        self.ctx.check(_lib.lib().pargo_idw_run(self.ctx.h, self.plan, self.k, self.power, self.buffer,
+                                                _lib._ptr(self.mask), mr, mc, self.mask_nodata, _lib._ptr(out),
+                                                _lib._ptr(cand)))
+        if count_candidates:
+            return out, cand
+        return out
+
+    def adjoint(self, grad_out):
+        """dL/d(sample values) in file order (float64 device tensor)."""
+        import torch
+        g = torch.empty(self.n, dtype=torch.float64, device=grad_out.device)
+        mr = mc = 0
+        if self.mask is not None:
+            mr, mc = self.mask.shape
+        self.ctx.bind_stream()
+        self.ctx.check(_lib.lib().pargo_idw_adjoint(self.ctx.h, self.plan, self.k, self.power, self.buffer,
+                                                    _lib._ptr(self.mask), mr, mc, self.mask_nodata,
+                                                    _lib._ptr(grad_out.contiguous()), _lib._ptr(g)))
+        return g
+
+    def free(self):
+        if self.plan:
+            _lib.lib().pargo_idw_plan_free(self.ctx.h, self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,195 @@
+"""GPU parity of slope / D8 against the FP64 oracle (oracle/pargo_oracle.c).
+
+Bar: D8 codes and NoData masks bit-exact; slope within 1e-5 relative
+(|gpu - oracle| <= 1e-5 * |oracle| + 1e-12), max and RMS relative error
+printed.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from pargo_amd import focal, synth
+
+pytestmark = pytest.mark.gpu
+
+SLOPE_RTOL = 1e-5
+SLOPE_ATOL = 1e-12
+
+
+def upload(z, pitch_pad=0, col_offset=0):
+    import torch
+    rows, cols = z.shape
+    pitch = cols + col_offset + pitch_pad
+    buf = torch.zeros((rows, pitch), dtype=torch.float32, device="cuda")
+    buf[:, col_offset:col_offset + cols] = torch.from_numpy(np.ascontiguousarray(z, dtype=np.float32)).cuda()
+    return buf[:, col_offset:col_offset + cols]
+
+
+def gpu_run(z, cs, nodata, method="FD", dirtype=256, unit="tan", pitch_pad=0, col_offset=0):
+    t = upload(z, pitch_pad, col_offset)
+    s, c, m = focal.slope_d8(t, cs, nodata, method, unit, dirtype, want_mask=True)
+    import torch
+    torch.cuda.synchronize()
+    return s.cpu().numpy(), c.cpu().numpy(), focal.unpack_mask(m, z.shape[1]).cpu().numpy()
+
+
+def ref_codes(ref_d8, nodata, dirtype):
+    """Oracle double layer -> (device code, NoData mask)."""
+    nd = ref_d8 == nodata
+    flat = ref_d8 == -1
+    code = np.where(flat, focal.PARGO_FLAT[dirtype], ref_d8)
+    code = np.where(nd, 0, code).astype(np.int64)
+    return code, nd
+
+
+def check_case(z, cs=30.0, nodata=-9999.0, method="FD", dirtype=256, unit="tan", **kw):
+    gs, gc, gm = gpu_run(z, cs, nodata, method, dirtype, unit, **kw)
+    ref_s = pyoracle.slope(z, cs, nodata, method)
+    ref_d = pyoracle.d8(z, nodata, dirtype)
+    code, nd8 = ref_codes(ref_d, nodata, dirtype)
+    # NoData mask of the fused call is the D8 NoData set (+ border).
+    assert np.array_equal(gm, nd8), "nodata mask mismatch at %s" % (np.argwhere(gm != nd8)[:5],)
+    got = np.where(nd8, 0, gc.astype(np.int64))
+    bad = np.argwhere(got != code)
+    assert bad.size == 0, "D8 mismatch at %s: gpu %s oracle %s" % (
+        bad[:5].tolist(), got[tuple(bad[:5].T)], code[tuple(bad[:5].T)])
+    nds = ref_s == nodata
+    assert np.all(gs[nds] == np.float32(nodata))
+    r = ref_s[~nds]
+    if unit == "degrees":
+        r = np.degrees(np.arctan(r))
+    g = gs[~nds].astype(np.float64)
+    fin = np.isfinite(r)
+    assert np.array_equal(np.isfinite(g), fin)
+    assert np.array_equal(np.isnan(g), np.isnan(r))
+    r, g = r[fin], g[fin]
+    err = np.abs(g - r)
+    ok = err <= SLOPE_RTOL * np.abs(r) + SLOPE_ATOL
+    rel = err / np.maximum(np.abs(r), 1e-30)
+    if r.size:
+        print("slope %s: n=%d max_rel=%.3e rms_rel=%.3e" % (method, r.size, rel[r != 0].max(initial=0),
+                                                           np.sqrt(np.mean(rel[r != 0] ** 2)) if (r != 0).any() else 0))
+    assert ok.all(), "slope mismatch: worst rel %.3e" % rel.max()
+
+
+def terrain(rows, cols, seed=synth.BASE_SEED + 11):
+    z = synth.dem(seed, rows, cols)
+    synth.add_plateaus(seed, z, 0.02)
+    synth.add_nodata_discs(seed, z, fraction=0.01)
+    return z
+
+
+def test_terrain_odd_size():
+    check_case(terrain(515, 777))
+
+
+def test_terrain_1k():
+    check_case(synth.workload_dem("slope1k"))
+
+
+@pytest.mark.parametrize("rows,cols", [(3, 3), (3, 8), (4, 5), (7, 3), (9, 257), (130, 1025), (66, 2051)])
+def test_small_and_ragged(rows, cols):
+    rng = np.random.default_rng(rows * 1000 + cols)
+    z = rng.uniform(100, 110, size=(rows, cols)).astype(np.float32)
+    check_case(z)
+
+
+@pytest.mark.parametrize("pad,off", [(4, 0), (13, 0), (0, 1), (3, 2)])
+def test_pitched_and_unaligned(pad, off):
+    check_case(terrain(301, 533, seed=7), pitch_pad=pad, col_offset=off)
+
+
+def test_epsilon_boundaries():
+    """Differences of 1, 2, 3 units of 2^-20 (0.95e-6, 1.9e-6, ...) straddle
+    the 1e-6 tie and flat tests for cardinal and diagonal neighbours."""
+    rng = np.random.default_rng(5)
+    for base in (0.0, 1.0, 3.0):
+        z = (base + rng.integers(0, 5, size=(200, 300)) * 2.0 ** -20).astype(np.float32)
+        check_case(z, nodata=-9999.0)
+        check_case(z, nodata=-9999.0, dirtype=8)
+
+
+def test_integer_terrain_ties():
+    rng = np.random.default_rng(6)
+    z = rng.integers(0, 4, size=(257, 300)).astype(np.float32) * 10
+    check_case(z)
+    check_case(z, dirtype=8)
+
+
+def test_diagonal_cardinal_near_ties():
+    # cardinal drop h against diagonal drop ~sqrt(2) h at float resolution
+    rng = np.random.default_rng(8)
+    h = rng.uniform(0.5, 5, size=(180, 240))
+    z = np.zeros((180, 240), dtype=np.float32)
+    z[::2, ::2] = 1000.0
+    z[1::2, ::2] = (1000.0 - h[1::2, ::2]).astype(np.float32)
+    z[::2, 1::2] = (1000.0 - np.sqrt(2) * h[::2, 1::2]).astype(np.float32)
+    z[1::2, 1::2] = (1000.0 - np.sqrt(2) * h[1::2, 1::2]).astype(np.float32)
+    check_case(z)
+    check_case(z, dirtype=8)
+
+
+def test_nonfinite_values():
+    z = terrain(200, 300, seed=9)
+    z[50, 50] = np.nan
+    z[100, 7] = np.inf
+    z[120, 299] = -np.inf
+    z[0, 10] = np.nan
+    check_case(z)
+
+
+@pytest.mark.parametrize("method", ["FD", "FFD", "MD", "SD", "SFD", "TFD", "TFDW"])
+def test_slope_methods(method):
+    check_case(terrain(211, 309, seed=13), method=method)
+
+
+def test_degrees():
+    check_case(terrain(211, 309, seed=14), unit="degrees")
+
+
+def test_dirtype8_terrain():
+    check_case(terrain(300, 401, seed=15), dirtype=8)
+
+
+@pytest.mark.parametrize("nodata", [0.0, 3.0000001, -9999.0, 1e30])
+def test_other_nodata_values(nodata):
+    rng = np.random.default_rng(16)
+    z = rng.integers(0, 6, size=(150, 220)).astype(np.float32)
+    z[rng.random(z.shape) < 0.03] = np.float32(nodata)
+    z[10:20, 10:20] = np.float32(3.0000002)
+    check_case(z, cs=1.0, nodata=nodata)
+
+
+def test_d8_only_and_slope_only_entry_points():
+    import torch
+    z = terrain(257, 263, seed=17)
+    t = upload(z)
+    s = focal.slope(t, 30.0, -9999.0)
+    c = focal.d8(t, 30.0, -9999.0)
+    s2, c2, _ = focal.slope_d8(t, 30.0, -9999.0)
+    torch.cuda.synchronize()
+    assert torch.equal(c, c2)
+    assert torch.equal(s, s2)
+
+
+def test_large_4k():
+    z = terrain(4096, 4096, seed=synth.BASE_SEED + 1)
+    gs, gc, gm = gpu_run(z, 30.0, -9999.0)
+    ref_d = pyoracle.d8(z, -9999.0, 256, nthreads=16)
+    code, nd8 = ref_codes(ref_d, -9999.0, 256)
+    assert np.array_equal(gm, nd8)
+    assert np.array_equal(np.where(nd8, 0, gc.astype(np.int64)), code)
+    ref_s = pyoracle.slope(z, 30.0, -9999.0, "FD", nthreads=16)
+    v = ref_s != -9999.0
+    rel = np.abs(gs[v] - ref_s[v]) / np.maximum(np.abs(ref_s[v]), 1e-30)
+    assert np.all(np.abs(gs[v] - ref_s[v]) <= SLOPE_RTOL * np.abs(ref_s[v]) + SLOPE_ATOL), rel.max()
+
+
+def test_repeatable_bitwise():
+    import torch
+    z = terrain(300, 300, seed=18)
+    t = upload(z)
+    a = focal.slope_d8(t, 30.0, -9999.0)
+    b = focal.slope_d8(t, 30.0, -9999.0)
+    torch.cuda.synchronize()
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])

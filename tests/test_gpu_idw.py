@@ -1,0 +1,124 @@
+"""GPU parity of IDW against the FP64 oracle (oracle/pargo_oracle.c).
+
+Bar: |gpu - oracle| <= 1e-5 * |oracle| + 1e-9 on every cell (float32
+output of an FP64 computation); masked cells exactly -9999.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from pargo_amd import _lib, synth
+from pargo_amd.idw import IDWOperator
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+ATOL = 1e-9
+
+
+def run_gpu(xs, ys, vs, cs, k, p, buffer, block, mask=None, mask_nodata=0):
+    import torch
+    op = IDWOperator(cs, k, p, buffer, block).read_samples(xs, ys, vs)
+    if mask is not None:
+        op.mask_layer(torch.from_numpy(np.ascontiguousarray(mask, dtype=np.int32)).cuda(), mask_nodata)
+    out, cand = op.run(count_candidates=True)
+    torch.cuda.synchronize()
+    res = out.cpu().numpy().astype(np.float64), int(cand.item())
+    op.free()
+    return res
+
+
+def compare(xs, ys, vs, cs, k=12, p=2, buffer=0.0, block=5000.0, mask=None, mask_nodata=0, nthreads=16):
+    got, cand = run_gpu(xs, ys, vs, cs, k, p, buffer, block, mask, mask_nodata)
+    ref, rcand, _ = pyoracle.idw(xs, ys, vs, cs, k=k, power=p, buffer=buffer, block_size=block, mask=mask,
+                                 mask_nodata=mask_nodata, nthreads=nthreads, stats=True)
+    assert got.shape == ref.shape
+    err = np.abs(got - ref)
+    rel = err / np.maximum(np.abs(ref), 1e-30)
+    print("idw k=%d p=%d buffer=%g: max_rel=%.3e rms_rel=%.3e cand=%d" % (k, p, buffer, rel.max(),
+                                                                      np.sqrt(np.mean(rel ** 2)), cand))
+    assert np.all(err <= RTOL * np.abs(ref) + ATOL), "worst rel %.3e at %s" % (rel.max(), np.unravel_index(
+        rel.argmax(), rel.shape))
+    assert cand == int(rcand.sum())
+    return got
+
+
+def rand_samples(n, seed, extent):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, extent, n), rng.uniform(0, extent, n), rng.uniform(100, 200, n)
+
+
+@pytest.mark.parametrize("k,p", [(1, 2), (5, 1), (12, 2), (12, 3), (16, 2), (32, 2)])
+def test_knn_buffer_zero(k, p):
+    xs, ys, vs = rand_samples(400, k * 10 + p, 6000.0)
+    compare(xs, ys, vs, 37.0, k, p, 0.0, 700.0)
+
+
+@pytest.mark.parametrize("buffer", [150.0, 600.0, 2500.0])
+def test_buffer_rule(buffer):
+    xs, ys, vs = rand_samples(120, 3, 6000.0)
+    got = compare(xs, ys, vs, 40.0, 12, 2, buffer, 500.0)
+    if buffer == 150.0:
+        assert (got == 0).any()
+
+
+def test_synthetic_samples_small_grid():
+    # the benchmark's sample recipe on a 512 x 512 grid (15,360 m square)
+    xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 2500, width=15360.0, height=15360.0)
+    compare(xs, ys, vs, 30.0, 12, 2, 5000.0, 5000.0)
+    compare(xs, ys, vs, 30.0, 12, 2, 0.0, 1000.0)
+
+
+def test_reference_tie_layouts():
+    # strict-< replacement keeps the first visited of equal distances and
+    # evicts the first slot holding the maximum (hand-checked on the oracle)
+    base_x, base_y, base_v = [0.5, 999.5], [0.5, 999.5], [-1.0, -1.0]
+    a, b, c = (555.5, 450.5, 10.0), (545.5, 450.5, 20.0), (550.5, 453.5, 40.0)
+    for extra, k in (([a, b], 1), ([b, a], 1), ([a, b, c], 2), ([c, a, b], 2)):
+        xs = np.array(base_x + [e[0] for e in extra])
+        ys = np.array(base_y + [e[1] for e in extra])
+        vs = np.array(base_v + [e[2] for e in extra])
+        got = compare(xs, ys, vs, 1.0, k, 2, 0.0, 100.0)
+        ref = pyoracle.idw(xs, ys, vs, 1.0, k=k, power=2, block_size=100.0, nthreads=16)
+        assert got[549, 550] == pytest.approx(ref[549, 550], rel=1e-7)
+
+
+def test_grid_lattice_ties():
+    # samples on a lattice: many exactly equal distances
+    g = np.arange(5, 500, 20.0)
+    xs, ys = np.meshgrid(g, g)
+    xs, ys = xs.ravel(), ys.ravel()
+    vs = np.arange(len(xs), dtype=np.float64) % 17
+    compare(xs, ys, vs, 10.0, 4, 2, 0.0, 100.0)
+    compare(xs, ys, vs, 10.0, 12, 2, 45.0, 100.0)
+
+
+def test_coincident_samples():
+    xs = np.array([15.0, 75.0, 45.0, 45.0, 105.0])
+    ys = np.array([15.0, 75.0, 45.0, 45.0, 15.0])
+    vs = np.array([1.0, 3.0, 7.0, 9.0, 2.0])
+    compare(xs, ys, vs, 30.0, 3, 2, 0.0, 60.0)
+
+
+def test_mask():
+    xs, ys, vs = rand_samples(200, 9, 3000.0)
+    rows, cols, _, _ = pyoracle.idw_dims(xs, ys, 25.0)
+    rng = np.random.default_rng(10)
+    mask = rng.integers(0, 3, size=(rows // 3 + 1, cols // 2 + 1)).astype(np.int32)
+    got = compare(xs, ys, vs, 25.0, 6, 2, 0.0, 400.0, mask=mask, mask_nodata=2)
+    assert (got == -9999.0).any()
+
+
+def test_nonterminating_request_is_rejected():
+    xs, ys, vs = rand_samples(5, 11, 300.0)
+    op = IDWOperator(10.0, 12, 2, 0.0, 100.0).read_samples(xs, ys, vs)
+    with pytest.raises(_lib.PargoError) as e:
+        op.run()
+    assert e.value.code == _lib.PARGO_E_NONTERM
+    op.free()
+
+
+def test_idw_medium_full_recipe():
+    # 1024 x 1024 grid of the benchmark recipe (30 m cells, 5 km blocks/buffer)
+    xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 5000, width=30720.0, height=30720.0)
+    compare(xs, ys, vs, 30.0, 12, 2, 5000.0, 5000.0)
