@@ -27,6 +27,7 @@
 //
 // The Horn numerators are formed in FP64 (exact for float32 input of
 // moderate range), the square root in FP32: relative error below 5e-7.
+#include <algorithm>
 #include <cmath>
 
 #include "focal_common.h"
@@ -72,6 +73,7 @@ struct FocalParams {
     int mask_words;       // uint32 words per mask row
     float nd_lo, nd_hi;   // slope NoData interval (eps 1e-7)
     float nd8_lo, nd8_hi; // D8 NoData interval (eps 1e-6)
+    float ndu_lo, ndu_hi; // smallest interval holding both (+inf,-inf if empty)
     float slope_nodata;   // value written to NoData slope cells
     double inv64cs2;      // 1 / (64 cs^2) for the Horn fast path
     double cs;            // float-rounded cell size
@@ -79,24 +81,32 @@ struct FocalParams {
     int unit;
     int dirtype;          // 0: no D8, 8 or 256
     int mask_from_d8;     // NoData mask reports the D8 NoData set
+    // Cells whose window holds Inf or |z| >= 2^40 are queued here and
+    // recomputed by focal_fixup_kernel with the reference-order FP64 code.
+    unsigned *fix_count;  // [0] queued cells, [1] finished fixup blocks
+    unsigned *fix_list;   // linear cell indices r * cols + c
+    unsigned fix_cap;
 };
 
 template <int CPT>
 struct RawRow {
     float v[CPT];
-    float hl, hr;  // outer halo, valid on lanes 0 and 63
+    float hl;  // outer halo column: c0-1 on lane 0, c0+CPT on lane 63
 };
 
+// `full`: the whole wave's columns are inside the raster (wave-uniform).
+// Every lane issues the halo load; lane 0 fetches column c0-1, lane 63
+// column c0+CPT and the others re-read a column of their own segment (an L1
+// hit), which is cheaper than branching on the lane id.
 template <int CPT, bool VEC>
-__device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, int lane, RawRow<CPT> &o)
+__device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, int lane, bool full, RawRow<CPT> &o)
 {
 #pragma unroll
     for (int k = 0; k < CPT; k++) o.v[k] = 0.f;
     o.hl = 0.f;
-    o.hr = 0.f;
     if (r < 0 || r >= p.rows) return;  // wave-uniform
     const float *rowp = p.dem + (long long)r * p.pitch;
-    if (VEC && c0 + CPT <= p.cols) {
+    if (VEC && full) {
         if constexpr (CPT == 4) {
             float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
             o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
@@ -112,43 +122,106 @@ __device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, in
         for (int k = 0; k < CPT; k++)
             if (c0 + k < p.cols) o.v[k] = rowp[c0 + k];
     }
-    if (lane == 0 && c0 >= 1 && c0 - 1 < p.cols) o.hl = rowp[c0 - 1];
-    if (lane == 63 && c0 + CPT < p.cols) o.hr = rowp[c0 + CPT];
+    int hc = (lane == 0) ? c0 - 1 : (lane == 63 ? c0 + CPT : c0);
+    hc = hc < 0 ? 0 : (hc > p.cols - 1 ? p.cols - 1 : hc);
+    o.hl = rowp[hc];
 }
 
-// Values of columns c0-1 .. c0+CPT as doubles, with NoData and "special"
-// flags per position (bit i for position i).
+// Cross-lane moves by DPP wavefront shifts (lane i reads lane i-1 / i+1).
+__device__ __forceinline__ float from_lower_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_upper_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// FP64 max/min without the quieting "v_max x,x" LLVM inserts for values it
+// cannot prove canonical (loop-carried ones): every operand here is the
+// result of an FP64 add/fma, so it is never a signalling NaN, and the
+// instructions' IEEE maxNum/minNum semantics (a quiet NaN operand yields the
+// other operand) are exactly what the reference's "if (diff > max)" does.
+__device__ __forceinline__ double vmax(double a, double b)
+{
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double vmin(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// max(a, -b, 0)
+__device__ __forceinline__ double vmax_neg_zero(double a, double b)
+{
+    double r;
+    asm("v_max_f64 %0, %1, -%2\n\tv_max_f64 %0, %0, 0" : "=&v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// c = 2c + (lane bit of `mask`): one v_addc with the lane mask as carry-in.
+__device__ __forceinline__ unsigned push_bit(unsigned c, bool cond)
+{
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(cond);
+    unsigned r;
+    asm("v_addc_co_u32_e64 %0, vcc, %1, %1, %2" : "=v"(r) : "v"(c), "s"(m) : "vcc");
+    return r;
+}
+
+// One DEM row of the rolling window (columns c0-1 .. c0+CPT) and, once the
+// row below is known, the drops from this row to it.  A kernel keeps three
+// slots and rotates their roles statically, so nothing is copied per row.
 template <int CPT>
-struct Row {
-    double z[CPT + 2];
-    unsigned nd;   // slope NoData interval
-    unsigned nd8;  // D8 NoData interval
-    unsigned sp;   // NaN, Inf or |z| >= 2^40
+struct Slot {
+    double z[CPT + 2];   // the row; replaced by z + 2*below once it is "up"
+    double V[CPT];       // RN(z[j+1] - below[j+1])
+    double SE[CPT + 1];  // G(z[i] - below[i+1]), i = 0..CPT
+    double SW[CPT + 2];  // G(z[i] - below[i-1]), i = 1..CPT+1
+    bool exc;            // NoData or a special value among this lane's values
+    unsigned nd, nd8, sp;  // per-position bits, filled only when exc
 };
 
+// Converts a loaded row.  The common case costs a min, a max and a max of
+// magnitudes over the CPT+2 values; only rows holding NoData or values with
+// |z| >= 2^40 (incl. Inf) get per-position bits.  NaN needs no flag: the
+// FP64 max and tie tests skip NaN drops exactly as the reference's > and <
+// comparisons do, and NaN is never NoData.
 template <int CPT>
-__device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CPT> &raw, int lane, Row<CPT> &o)
+__device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CPT> &raw, int lane, Slot<CPT> &o)
 {
     float f[CPT + 2];
-    float left = __shfl_up(raw.v[CPT - 1], 1, 64);
-    float right = __shfl_down(raw.v[0], 1, 64);
+    const float left = from_lower_lane(raw.v[CPT - 1]);
+    const float right = from_upper_lane(raw.v[0]);
     f[0] = (lane == 0) ? raw.hl : left;
-    f[CPT + 1] = (lane == 63) ? raw.hr : right;
+    f[CPT + 1] = (lane == 63) ? raw.hl : right;
 #pragma unroll
     for (int k = 0; k < CPT; k++) f[k + 1] = raw.v[k];
-    unsigned nd = 0, nd8 = 0, sp = 0;
+    float mn = f[0], mx = f[0], am = __builtin_fabsf(f[0]);
 #pragma unroll
-    for (int i = 0; i < CPT + 2; i++) {
-        o.z[i] = (double)f[i];
-        nd |= (unsigned)in_interval(f[i], p.nd_lo, p.nd_hi) << i;
-        nd8 |= (unsigned)in_interval(f[i], p.nd8_lo, p.nd8_hi) << i;
-        // NaN, Inf and |z| >= 2^40 take the per-cell FP64 path: the fast
-        // path's FP32 magnitude stage would overflow for such values.
-        sp |= (unsigned)(!(__builtin_fabsf(f[i]) < 0x1p40f)) << i;
+    for (int i = 1; i < CPT + 2; i++) {
+        mn = fminf(mn, f[i]);
+        mx = fmaxf(mx, f[i]);
+        am = fmaxf(am, __builtin_fabsf(f[i]));
     }
-    o.nd = nd;
-    o.nd8 = nd8;
-    o.sp = sp;
+    o.exc = !(mx < p.ndu_lo || mn > p.ndu_hi) || !(am < 0x1p40f);
+    o.nd = o.nd8 = o.sp = 0;
+    if (o.exc) {
+        unsigned nd = 0, nd8 = 0, sp = 0;
+#pragma unroll
+        for (int i = 0; i < CPT + 2; i++) {
+            nd |= (unsigned)in_interval(f[i], p.nd_lo, p.nd_hi) << i;
+            nd8 |= (unsigned)in_interval(f[i], p.nd8_lo, p.nd8_hi) << i;
+            sp |= (unsigned)(!(__builtin_fabsf(f[i]) < 0x1p40f)) << i;
+        }
+        o.nd = nd;
+        o.nd8 = nd8;
+        o.sp = sp;
+    }
+#pragma unroll
+    for (int i = 0; i < CPT + 2; i++) o.z[i] = (double)f[i];
 }
 
 // Per-cell slope for any method, in the reference's FP64 operation order
@@ -221,6 +294,180 @@ __device__ __forceinline__ float slope_unit(float s, int unit)
     return unit == PARGO_UNIT_DEGREES ? atanf(s) * 57.295779513082323f : s;
 }
 
+// Output row r with up = row r-1, mid = row r, dn = the slot that receives
+// row r+1 from `raw`; `raw` is then refilled with row r+4.
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+__device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, int lane, bool full, unsigned colb,
+                                           Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw)
+{
+    finish_row<CPT>(p, raw, lane, dn);
+    load_raw<CPT, VEC>(p, r + 4, c0, lane, full, raw);
+
+    float sl[CPT];
+    unsigned cd[CPT];
+    // ---- slope ------------------------------------------------------------
+    if (DO_SLOPE) {
+        if (HORN) {
+            // Horn numerators: nx from column sums up + 2 mid + dn, ny from the
+            // vertical differences up - dn (both FP64; the float inputs make
+            // them exact or within an ulp of the reference's sums)
+            double col[CPT + 2], vd[CPT + 2];
+#pragma unroll
+            for (int i = 0; i < CPT + 2; i++) {
+                col[i] = __builtin_fma(2.0, mid.z[i], dn.z[i]) + up.z[i];
+                vd[i] = up.z[i] - dn.z[i];
+            }
+#pragma unroll
+            for (int j = 0; j < CPT; j++) {
+                const double nx = col[j + 2] - col[j];
+                const double ny = __builtin_fma(2.0, vd[j + 1], vd[j] + vd[j + 2]);
+                const double q = __builtin_fma(nx, nx, ny * ny) * p.inv64cs2;
+                sl[j] = __builtin_amdgcn_sqrtf((float)q);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; j++) {
+                const int i = j + 1;
+                const double d[9] = {up.z[i - 1], up.z[i], up.z[i + 1], mid.z[i - 1], mid.z[i],
+                                     mid.z[i + 1], dn.z[i - 1], dn.z[i], dn.z[i + 1]};
+                sl[j] = (float)slope_generic(d, p.method, p.cs);
+            }
+        }
+    }
+    // ---- D8 ---------------------------------------------------------------
+    if (DO_D8) {
+        double H[CPT + 1];
+#pragma unroll
+        for (int i = 0; i <= CPT; i++) H[i] = mid.z[i] - mid.z[i + 1];
+#pragma unroll
+        for (int j = 0; j < CPT; j++) mid.V[j] = mid.z[j + 1] - dn.z[j + 1];
+#pragma unroll
+        for (int i = 0; i <= CPT; i++) mid.SE[i] = div_sqrt2(mid.z[i] - dn.z[i + 1]);
+#pragma unroll
+        for (int i = 1; i <= CPT + 1; i++) mid.SW[i] = div_sqrt2(mid.z[i] - dn.z[i - 1]);
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int i = j + 1;
+            // drops  E=H[i]  W=-H[i-1]  S=mid.V[j]  N=-up.V[j]
+            //        SE=mid.SE[i]  NW=-up.SE[i-1]  SW=mid.SW[i]  NE=-up.SW[i+1]
+            // max(0, drops) = max(0, max(E,S,SE,SW), -min(-W,-N,-NW,-NE))
+            const double a = vmax(vmax(H[i], mid.V[j]), vmax(mid.SE[i], mid.SW[i]));
+            const double b = vmin(vmin(H[i - 1], up.V[j]), vmin(up.SE[i - 1], up.SW[i + 1]));
+            const double M = vmax_neg_zero(a, b);
+            unsigned c = 0;
+            c = push_bit(c, fabs(-up.SW[i + 1] - M) < kD8Eps);  // NE  128
+            c = push_bit(c, fabs(-up.V[j] - M) < kD8Eps);       // N    64
+            c = push_bit(c, fabs(-up.SE[i - 1] - M) < kD8Eps);  // NW   32
+            c = push_bit(c, fabs(-H[i - 1] - M) < kD8Eps);      // W    16
+            c = push_bit(c, fabs(mid.SW[i] - M) < kD8Eps);      // SW    8
+            c = push_bit(c, fabs(mid.V[j] - M) < kD8Eps);       // S     4
+            c = push_bit(c, fabs(mid.SE[i] - M) < kD8Eps);      // SE    2
+            c = push_bit(c, fabs(H[i] - M) < kD8Eps);           // E     1
+            cd[j] = (M < kD8Eps) ? 0u : c;
+        }
+    }
+
+    // ---- NoData, border and special windows (rare) -------------------------
+    // (the queued special cells keep their fast-path values until the fixup)
+    const bool rowb = (r == 0) || (r == p.rows - 1);
+    const bool wexc = up.exc | mid.exc | dn.exc | (colb != 0) | rowb;
+    unsigned ndflags = 0;  // bit j: slope NoData; bit j+8: D8 NoData
+    if (__builtin_expect(wexc, 0)) {
+        const unsigned spw = up.sp | mid.sp | dn.sp;
+        const unsigned ndw = up.nd | mid.nd | dn.nd;
+        const unsigned nd8w = up.nd8 | mid.nd8 | dn.nd8;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const bool border = rowb || ((colb >> j) & 1u);
+            const bool nds = border || ((ndw >> j) & 7u), n8 = border || ((nd8w >> j) & 7u);
+            ndflags |= (unsigned)nds << j;
+            ndflags |= (unsigned)n8 << (j + 8);
+            // a special value in a window that is not NoData for every output:
+            // queue the cell for the exact pass
+            if (((spw >> j) & 7u) && !((!DO_SLOPE || nds) && (!DO_D8 || n8))) {
+                const unsigned k = atomicAdd(p.fix_count, 1u);
+                if (k < p.fix_cap) p.fix_list[k] = (unsigned)r * (unsigned)p.cols + (unsigned)(c0 + j);
+            }
+        }
+    }
+    if (DO_SLOPE && p.unit == PARGO_UNIT_DEGREES) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) sl[j] = slope_unit(sl[j], p.unit);
+    }
+    unsigned ndbits = 0;  // NoData mask bits of this lane's cells
+    if (__builtin_expect(ndflags != 0, 0)) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const bool nds = (ndflags >> j) & 1u, n8 = (ndflags >> (j + 8)) & 1u;
+            if (DO_SLOPE && nds) sl[j] = p.slope_nodata;
+            if (DO_D8 && n8) cd[j] = 0;
+            const bool m = (DO_D8 && p.mask_from_d8) ? n8 : nds;
+            ndbits |= (unsigned)(m && (c0 + j) < p.cols) << j;
+        }
+    }
+    if (DO_D8 && p.dirtype == 8) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) cd[j] = cd[j] ? (unsigned)c_lut8.v[cd[j]] : 255u;
+    }
+
+    // ---- stores --------------------------------------------------------------
+    const long long base = (long long)r * p.pitch + c0;
+    if (DO_SLOPE) {
+        if (VEC && full) {
+            if constexpr (CPT == 4) {
+                *reinterpret_cast<float4 *>(p.slope + base) = make_float4(sl[0], sl[1], sl[2], sl[3]);
+            } else if constexpr (CPT == 2) {
+                *reinterpret_cast<float2 *>(p.slope + base) = make_float2(sl[0], sl[1]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < CPT; j++) p.slope[base + j] = sl[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; j++)
+                if (c0 + j < p.cols) p.slope[base + j] = sl[j];
+        }
+    }
+    if (DO_D8) {
+        if (VEC && full) {
+            if constexpr (CPT == 4) {
+                *reinterpret_cast<unsigned *>(p.code + base) = cd[0] | (cd[1] << 8) | (cd[2] << 16) | (cd[3] << 24);
+            } else if constexpr (CPT == 2) {
+                *reinterpret_cast<unsigned short *>(p.code + base) = (unsigned short)(cd[0] | (cd[1] << 8));
+            } else {
+#pragma unroll
+                for (int j = 0; j < CPT; j++) p.code[base + j] = (uint8_t)cd[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; j++)
+                if (c0 + j < p.cols) p.code[base + j] = (uint8_t)cd[j];
+        }
+    }
+    if (p.mask) {
+        // gather the CPT-bit groups of 32/CPT lanes into one word
+        constexpr int LPW = 32 / CPT;
+        unsigned w = ndbits << (CPT * (lane % LPW));
+#pragma unroll
+        for (int o = 1; o < LPW; o <<= 1) w |= __shfl_xor(w, o, 64);
+        if ((lane % LPW) == 0 && c0 < p.cols) p.mask[(long long)r * p.mask_words + (c0 >> 5)] = w;
+    }
+}
+
+// Pairs from a row to the row below (the prologue of a strip).
+template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+__device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
+{
+    if (DO_D8) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) up.V[j] = up.z[j + 1] - below.z[j + 1];
+#pragma unroll
+        for (int i = 0; i <= CPT; i++) up.SE[i] = div_sqrt2(up.z[i] - below.z[i + 1]);
+#pragma unroll
+        for (int i = 1; i <= CPT + 1; i++) up.SW[i] = div_sqrt2(up.z[i] - below.z[i - 1]);
+    }
+}
+
 template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
 __global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
 {
@@ -228,212 +475,88 @@ __global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
     const int c0 = (blockIdx.x * blockDim.x + threadIdx.x) * CPT;
     const int rbeg = blockIdx.y * p.rows_per_strip;
     const int rend = min(rbeg + p.rows_per_strip, p.rows);
-
-    RawRow<CPT> ra, rb, rc;
-    load_raw<CPT, VEC>(p, rbeg - 1, c0, lane, ra);
-    load_raw<CPT, VEC>(p, rbeg, c0, lane, rb);
-    load_raw<CPT, VEC>(p, rbeg + 1, c0, lane, rc);  // first "down" row
-    RawRow<CPT> pf;
-    load_raw<CPT, VEC>(p, rbeg + 2, c0, lane, pf);  // prefetch
-
-    Row<CPT> up, mid;
-    finish_row<CPT>(p, ra, lane, up);
-    finish_row<CPT>(p, rb, lane, mid);
-
-    // Drops between the rows above and at the output row (see header):
-    //   Vp[j]  = RN(up[j] - mid[j])              j = 0..CPT-1  (position j+1)
-    //   SEp[i] = G(up[i] - mid[i+1])             i = 0..CPT
-    //   SWp[i] = G(up[i] - mid[i-1])             i = 1..CPT+1
-    double Vp[CPT], SEp[CPT + 1], SWp[CPT + 2];
-    if (DO_D8) {
+    // first column of the wave, read from lane 0 so that it is wave-uniform
+    const int wave_c0 = __builtin_amdgcn_readfirstlane(c0);
+    const bool full = wave_c0 + 64 * CPT <= p.cols;
+    unsigned colb = 0;  // bit j: column c0+j is on (or past) the border
 #pragma unroll
-        for (int j = 0; j < CPT; j++) Vp[j] = up.z[j + 1] - mid.z[j + 1];
-#pragma unroll
-        for (int i = 0; i <= CPT; i++) SEp[i] = div_sqrt2(up.z[i] - mid.z[i + 1]);
-#pragma unroll
-        for (int i = 1; i <= CPT + 1; i++) SWp[i] = div_sqrt2(up.z[i] - mid.z[i - 1]);
-        SWp[0] = 0.0;
+    for (int j = 0; j < CPT; j++) colb |= (unsigned)(c0 + j == 0 || c0 + j >= p.cols - 1) << j;
+
+    Slot<CPT> A, B, C;
+    RawRow<CPT> R0, R1, R2;
+    {
+        RawRow<CPT> ra, rb;
+        load_raw<CPT, VEC>(p, rbeg - 1, c0, lane, full, ra);
+        load_raw<CPT, VEC>(p, rbeg, c0, lane, full, rb);
+        load_raw<CPT, VEC>(p, rbeg + 1, c0, lane, full, R0);
+        load_raw<CPT, VEC>(p, rbeg + 2, c0, lane, full, R1);
+        load_raw<CPT, VEC>(p, rbeg + 3, c0, lane, full, R2);
+        finish_row<CPT>(p, ra, lane, A);
+        finish_row<CPT>(p, rb, lane, B);
     }
-    // Horn row sums (z[i-1] + z[i+1]) + 2 z[i] of the rows above / at.
-    double rsU[CPT], rsM[CPT];
-    if (DO_SLOPE && HORN) {
-#pragma unroll
-        for (int j = 0; j < CPT; j++) {
-            rsU[j] = __builtin_fma(2.0, up.z[j + 1], up.z[j] + up.z[j + 2]);
-            rsM[j] = __builtin_fma(2.0, mid.z[j + 1], mid.z[j] + mid.z[j + 2]);
-        }
-    }
-
-    for (int r = rbeg; r < rend; r++) {
-        Row<CPT> dn;
-        finish_row<CPT>(p, rc, lane, dn);
-        rc = pf;
-        load_raw<CPT, VEC>(p, r + 3, c0, lane, pf);
-
-        const bool row_border = (r == 0) || (r == p.rows - 1);
-        const unsigned ndw = up.nd | mid.nd | dn.nd;
-        const unsigned nd8w = up.nd8 | mid.nd8 | dn.nd8;
-        const unsigned spw = up.sp | mid.sp | dn.sp;
-
-        float sl[CPT];
-        unsigned cd[CPT];
-        unsigned ndbits = 0;  // bit j: cell j is NoData (for the mask)
-
-        // ---- D8 --------------------------------------------------------
-        double V[CPT], SE[CPT + 1], SW[CPT + 2];
-        if (DO_D8) {
-            double H[CPT + 1];
-#pragma unroll
-            for (int i = 0; i <= CPT; i++) H[i] = mid.z[i] - mid.z[i + 1];
-#pragma unroll
-            for (int j = 0; j < CPT; j++) V[j] = mid.z[j + 1] - dn.z[j + 1];
-#pragma unroll
-            for (int i = 0; i <= CPT; i++) SE[i] = div_sqrt2(mid.z[i] - dn.z[i + 1]);
-#pragma unroll
-            for (int i = 1; i <= CPT + 1; i++) SW[i] = div_sqrt2(mid.z[i] - dn.z[i - 1]);
-            SW[0] = 0.0;
-#pragma unroll
-            for (int j = 0; j < CPT; j++) {
-                const int i = j + 1;
-                const double E = H[i], W = -H[i - 1], S = V[j], N = -Vp[j];
-                const double dSE = SE[i], dNW = -SEp[i - 1], dSW = SW[i], dNE = -SWp[i + 1];
-                double M = fmax(fmax(fmax(E, W), fmax(S, N)), fmax(fmax(dSE, dNW), fmax(dSW, dNE)));
-                M = fmax(M, 0.0);
-                unsigned c = 0;
-                c = c + c + (unsigned)(fabs(dNE - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(N - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(dNW - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(W - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(dSW - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(S - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(dSE - M) < kD8Eps);
-                c = c + c + (unsigned)(fabs(E - M) < kD8Eps);
-                cd[j] = (M < kD8Eps) ? 0u : c;
-            }
-        }
-
-        // ---- slope -------------------------------------------------------
-        if (DO_SLOPE) {
-            if (HORN) {
-                double rsD[CPT];
-#pragma unroll
-                for (int j = 0; j < CPT; j++) rsD[j] = __builtin_fma(2.0, dn.z[j + 1], dn.z[j] + dn.z[j + 2]);
-                double col[CPT + 2];
-#pragma unroll
-                for (int i = 0; i < CPT + 2; i++) col[i] = __builtin_fma(2.0, mid.z[i], dn.z[i]) + up.z[i];
-#pragma unroll
-                for (int j = 0; j < CPT; j++) {
-                    const double nx = col[j + 2] - col[j];
-                    const double ny = rsU[j] - rsD[j];
-                    const double q = __builtin_fma(nx, nx, ny * ny) * p.inv64cs2;
-                    sl[j] = __builtin_amdgcn_sqrtf((float)q);
-                }
-#pragma unroll
-                for (int j = 0; j < CPT; j++) {
-                    rsU[j] = rsM[j];
-                    rsM[j] = rsD[j];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < CPT; j++) {
-                    const int i = j + 1;
-                    const double d[9] = {up.z[i - 1], up.z[i], up.z[i + 1], mid.z[i - 1], mid.z[i],
-                                         mid.z[i + 1], dn.z[i - 1], dn.z[i], dn.z[i + 1]};
-                    sl[j] = (float)slope_generic(d, p.method, p.cs);
-                }
-            }
-        }
-
-        // ---- windows with special values: per-cell IEEE path ---------------
-        if (__builtin_expect(__any(spw != 0), 0)) {
-#pragma unroll
-            for (int j = 0; j < CPT; j++) {
-                if ((spw >> j) & 7u) {
-                    const int i = j + 1;
-                    const double d[9] = {up.z[i - 1], up.z[i], up.z[i + 1], mid.z[i - 1], mid.z[i],
-                                         mid.z[i + 1], dn.z[i - 1], dn.z[i], dn.z[i + 1]};
-                    if (DO_D8) cd[j] = d8_exact(d);
-                    if (DO_SLOPE) sl[j] = (float)slope_generic(d, p.method, p.cs);
-                }
-            }
-        }
-
-        // ---- NoData, border, units, stores --------------------------------
-        const bool row_ok = r < p.rows;
-#pragma unroll
-        for (int j = 0; j < CPT; j++) {
-            const int c = c0 + j;
-            const bool border = row_border || c == 0 || c >= p.cols - 1;
-            const bool nds = border || ((ndw >> j) & 7u);
-            const bool nd8 = border || ((nd8w >> j) & 7u);
-            if (DO_SLOPE) sl[j] = nds ? p.slope_nodata : slope_unit(sl[j], p.unit);
-            if (DO_D8) {
-                unsigned v = nd8 ? 0u : cd[j];
-                if (p.dirtype == 8) v = (nd8 || v == 0) ? 255u : (unsigned)c_lut8.v[v];
-                cd[j] = v;
-            }
-            const bool m = (DO_D8 && p.mask_from_d8) ? nd8 : nds;
-            ndbits |= (unsigned)(m && c < p.cols) << j;
-        }
-        if (row_ok) {
-            const long long base = (long long)r * p.pitch + c0;
-            if (DO_SLOPE) {
-                if (VEC && c0 + CPT <= p.cols) {
-                    if constexpr (CPT == 4) {
-                        *reinterpret_cast<float4 *>(p.slope + base) = make_float4(sl[0], sl[1], sl[2], sl[3]);
-                    } else if constexpr (CPT == 2) {
-                        *reinterpret_cast<float2 *>(p.slope + base) = make_float2(sl[0], sl[1]);
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < CPT; j++) p.slope[base + j] = sl[j];
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < CPT; j++)
-                        if (c0 + j < p.cols) p.slope[base + j] = sl[j];
-                }
-            }
-            if (DO_D8) {
-                if (VEC && c0 + CPT <= p.cols) {
-                    if constexpr (CPT == 4) {
-                        const unsigned w = cd[0] | (cd[1] << 8) | (cd[2] << 16) | (cd[3] << 24);
-                        *reinterpret_cast<unsigned *>(p.code + base) = w;
-                    } else if constexpr (CPT == 2) {
-                        *reinterpret_cast<unsigned short *>(p.code + base) = (unsigned short)(cd[0] | (cd[1] << 8));
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < CPT; j++) p.code[base + j] = (uint8_t)cd[j];
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < CPT; j++)
-                        if (c0 + j < p.cols) p.code[base + j] = (uint8_t)cd[j];
-                }
-            }
-        }
-        if (p.mask) {
-            // Gather the CPT-bit groups of 32/CPT lanes into one word.
-            constexpr int LPW = 32 / CPT;
-            unsigned w = ndbits << (CPT * (lane % LPW));
-#pragma unroll
-            for (int o = 1; o < LPW; o <<= 1) w |= __shfl_xor(w, o, 64);
-            if (row_ok && (lane % LPW) == 0 && c0 < p.cols)
-                p.mask[(long long)r * p.mask_words + (c0 >> 5)] = w;
-        }
-
-        // ---- rotate --------------------------------------------------------
-        if (DO_D8) {
-#pragma unroll
-            for (int j = 0; j < CPT; j++) Vp[j] = V[j];
-#pragma unroll
-            for (int i = 0; i <= CPT; i++) SEp[i] = SE[i];
-#pragma unroll
-            for (int i = 0; i <= CPT + 1; i++) SWp[i] = SW[i];
-        }
-        up = mid;
-        mid = dn;
+    pair_rows<CPT, DO_SLOPE, DO_D8, HORN>(A, B);
+    // Three slots rotate roles statically: (up, mid, dn) = (A,B,C), (B,C,A), (C,A,B).
+    for (int r = rbeg; r < rend; r += 3) {
+        focal_step<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, r, c0, lane, full, colb, A, B, C, R0);
+        if (r + 1 >= rend) break;
+        focal_step<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, r + 1, c0, lane, full, colb, B, C, A, R1);
+        if (r + 2 >= rend) break;
+        focal_step<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, r + 2, c0, lane, full, colb, C, A, B, R2);
     }
 }
+
+// Recomputes queued cells with the reference-order FP64 code.  If the queue
+// overflowed it scans the whole grid for special windows instead.  The last
+// block to finish resets the queue for the next call on the stream.
+template <bool DO_SLOPE, bool DO_D8>
+__global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams p)
+{
+    const unsigned n = __atomic_load_n(&p.fix_count[0], __ATOMIC_RELAXED);
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    auto fix = [&](int r, int c, bool check) {
+        double d[9];
+        bool nd = false, nd8 = false, sp = false;
+        for (int k = 0; k < 9; k++) {
+            const float f = p.dem[(long long)(r + k / 3 - 1) * p.pitch + (c + k % 3 - 1)];
+            nd |= in_interval(f, p.nd_lo, p.nd_hi);
+            nd8 |= in_interval(f, p.nd8_lo, p.nd8_hi);
+            sp |= !(__builtin_fabsf(f) < 0x1p40f);
+            d[k] = (double)f;
+        }
+        if (check && !sp) return;
+        const long long o = (long long)r * p.pitch + c;
+        if (DO_D8 && !nd8) {
+            unsigned v = d8_exact(d);
+            if (p.dirtype == 8) v = v ? (unsigned)c_lut8.v[v] : 255u;
+            p.code[o] = (uint8_t)v;
+        }
+        if (DO_SLOPE && !nd) p.slope[o] = slope_unit((float)slope_generic(d, p.method, p.cs), p.unit);
+    };
+    if (n <= p.fix_cap) {
+        for (unsigned k = tid; k < n; k += stride) {
+            const unsigned cell = p.fix_list[k];
+            fix((int)(cell / (unsigned)p.cols), (int)(cell % (unsigned)p.cols), false);
+        }
+    } else {
+        const long long cells = (long long)(p.rows - 2) * (p.cols - 2);
+        for (long long k = tid; k < cells; k += stride)
+            fix(1 + (int)(k / (p.cols - 2)), 1 + (int)(k % (p.cols - 2)), true);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(&p.fix_count[1], 1u);
+        if (done == gridDim.x - 1) {
+            p.fix_count[0] = 0;
+            p.fix_count[1] = 0;
+            __threadfence();
+        }
+    }
+}
+
+constexpr int kFixupBlocks = 256;
+constexpr unsigned kFixupCap = 1u << 16;
 
 struct LaunchCfg {
     int cpt;
@@ -454,6 +577,14 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
         if (vec) hipLaunchKernelGGL((focal_kernel<2, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((focal_kernel<2, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
     }
+}
+
+static void launch_fixup(const FocalParams &p, bool do_slope, bool do_d8, hipStream_t s)
+{
+    dim3 block(256), grid(kFixupBlocks);
+    if (do_slope && do_d8) hipLaunchKernelGGL((focal_fixup_kernel<true, true>), grid, block, 0, s, p);
+    else if (do_slope) hipLaunchKernelGGL((focal_fixup_kernel<true, false>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p);
 }
 
 static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
@@ -477,8 +608,10 @@ static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, b
 static LaunchCfg default_cfg(int rows)
 {
     LaunchCfg c;
+    // 4 columns per lane and 32-row strips measured best for the fused
+    // kernel on 16384^2 (MI355X): 3 waves per SIMD, fewer halo drops per cell.
     c.cpt = 4;
-    c.rows_per_strip = 64;
+    c.rows_per_strip = 32;
     if (const char *e = getenv("PARGO_FOCAL_CPT")) c.cpt = atoi(e) == 2 ? 2 : 4;
     if (const char *e = getenv("PARGO_FOCAL_ROWS")) {
         int v = atoi(e);
@@ -486,6 +619,21 @@ static LaunchCfg default_cfg(int rows)
     }
     (void)rows;
     return c;
+}
+
+// The fixup queue: two counters and kFixupCap cell indices, zeroed once at
+// allocation; every fixup launch leaves the counters at zero again.
+static int ensure_fixup(pargo_ctx *ctx)
+{
+    if (ctx->fix_mem) return PARGO_OK;
+    const size_t bytes = (64 + (size_t)kFixupCap) * sizeof(unsigned);
+    hipError_t e = hipMalloc(&ctx->fix_mem, bytes);
+    if (e != hipSuccess) return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the fixup queue: %s", bytes,
+                                     hipGetErrorString(e));
+    e = hipMemsetAsync(ctx->fix_mem, 0, 64 * sizeof(unsigned), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, PARGO_E_DEVICE, "clearing the fixup queue: %s", hipGetErrorString(e));
+    return PARGO_OK;
 }
 
 // Validates the request and fills the kernel parameters.
@@ -519,6 +667,16 @@ static int prepare(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int me
     p.mask_words = (g->cols + 31) / 32;
     nodata_interval(g->nodata, 0.0000001, &p.nd_lo, &p.nd_hi);
     nodata_interval(g->nodata, 0.000001, &p.nd8_lo, &p.nd8_hi);
+    {
+        const bool e1 = p.nd_lo > p.nd_hi, e2 = p.nd8_lo > p.nd8_hi;
+        if (e1 && e2) {
+            p.ndu_lo = INFINITY;
+            p.ndu_hi = -INFINITY;
+        } else {
+            p.ndu_lo = e1 ? p.nd8_lo : (e2 ? p.nd_lo : std::min(p.nd_lo, p.nd8_lo));
+            p.ndu_hi = e1 ? p.nd8_hi : (e2 ? p.nd_hi : std::max(p.nd_hi, p.nd8_hi));
+        }
+    }
     p.slope_nodata = (float)g->nodata;
     p.cs = (double)csf;
     p.inv64cs2 = 1.0 / (64.0 * (double)csf * (double)csf);
@@ -547,7 +705,15 @@ static int run_focal(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int 
     const bool do_slope = slope_out != nullptr;
     const bool do_d8 = code_out != nullptr;
     if (!do_slope && !do_d8) return fail(ctx, PARGO_E_INVALID, "%s: a NoData mask needs a slope or code output", name);
+    rc = ensure_fixup(ctx);
+    if (rc) return rc;
+    p.fix_count = (unsigned *)ctx->fix_mem;
+    p.fix_list = (unsigned *)ctx->fix_mem + 64;
+    p.fix_cap = kFixupCap;
     launch_focal(p, cfg, vec, do_slope, do_d8, ctx->stream);
+    rc = check_launch(ctx, name);
+    if (rc) return rc;
+    launch_fixup(p, do_slope, do_d8, ctx->stream);
     return check_launch(ctx, name);
 }
 

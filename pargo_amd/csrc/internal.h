@@ -19,6 +19,7 @@ struct pargo_ctx {
     // Small reusable device scratch owned by the context (grows on demand).
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *fix_mem = nullptr;  // focal fixup queue (counters + cell list)
 };
 
 namespace pargo {
