@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-ffp-contract=off"]
 
-HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip"]
+HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip"]
 CXX_SOURCES = ["synth.cpp"]
 
 

@@ -1,0 +1,245 @@
+// adjoint.hip -- reverse-mode derivative of the Horn slope with respect to
+// the elevations (no counterpart in the reference, which has no gradients).
+//
+// Forward (slopeOperator.cpp:84-86):  s_i = |n_i| / (8 cs),
+//   nx_i = sum_d dc*(2-|dr|) z(i+d),  ny_i = sum_d -dr*(2-|dc|) z(i+d).
+// With p_i = g_i n_i / (8 cs |n_i|) (zero where slope is NoData, on the
+// border or where |n_i| = 0), the adjoint is the correlation with the
+// negated (antisymmetric) stencils:
+//   dL/dz(r,c) = [Px(c-1) - Px(c+1)] + [Qy(r+1) - Qy(r-1)],
+//   Px(k) = px(r-1,k) + 2 px(r,k) + px(r+1,k),
+//   Qy(k) = py(k,c-1) + 2 py(k,c) + py(k,c+1).
+// Only the DEM and dL/dslope are read; n_i is recomputed (12 bytes per
+// cell of HBM traffic: z in, g in, dL/dz out).
+//
+// Work split as in focal.hip: a lane owns 4 adjacent columns and walks a
+// strip of rows; it computes p for its columns plus one halo column on each
+// side (so it needs z two columns out), which keeps every wave independent.
+#include <cmath>
+
+#include "focal_common.h"
+#include "internal.h"
+
+namespace pargo {
+
+namespace {
+
+constexpr int ACPT = 4;       // columns per lane
+constexpr int AZW = ACPT + 4; // z columns c0-2 .. c0+5
+constexpr int APW = ACPT + 2; // p columns c0-1 .. c0+4
+
+struct AdjParams {
+    const float *__restrict__ dem;
+    const float *__restrict__ g;
+    float *__restrict__ gdem;
+    long long pitch;
+    int rows, cols, rows_per_strip;
+    float nd_lo, nd_hi;
+    double inv8cs;
+};
+
+__device__ __forceinline__ float lower_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float upper_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ int clampc(int c, int cols) { return c < 0 ? 0 : (c > cols - 1 ? cols - 1 : c); }
+
+struct ZRow {
+    double z[AZW];
+    unsigned nd;  // bit t: column c0-2+t holds NoData
+};
+
+// Row r of the DEM as doubles for columns c0-2 .. c0+5; zeros outside.
+template <bool VEC>
+__device__ __forceinline__ void load_z(const AdjParams &p, int r, int c0, int lane, bool full, ZRow &o)
+{
+    float v[ACPT] = {0.f, 0.f, 0.f, 0.f};
+    float h0 = 0.f, h1 = 0.f;
+    if (r >= 0 && r < p.rows) {
+        const float *rowp = p.dem + (long long)r * p.pitch;
+        if (VEC && full) {
+            const float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
+            v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < ACPT; k++)
+                if (c0 + k < p.cols) v[k] = rowp[c0 + k];
+        }
+        // outer halo: lane 0 needs c0-2, c0-1; lane 63 needs c0+4, c0+5
+        const int a = lane == 0 ? c0 - 2 : (lane == 63 ? c0 + ACPT : c0);
+        h0 = (a >= 0 && a < p.cols) ? rowp[a] : 0.f;
+        h1 = (a + 1 >= 0 && a + 1 < p.cols) ? rowp[a + 1] : 0.f;
+    }
+    float f[AZW];
+    const float l0 = lower_lane(v[2]), l1 = lower_lane(v[3]);
+    const float r0 = upper_lane(v[0]), r1 = upper_lane(v[1]);
+    f[0] = lane == 0 ? h0 : l0;
+    f[1] = lane == 0 ? h1 : l1;
+#pragma unroll
+    for (int k = 0; k < ACPT; k++) f[k + 2] = v[k];
+    f[6] = lane == 63 ? h0 : r0;
+    f[7] = lane == 63 ? h1 : r1;
+    unsigned nd = 0;
+#pragma unroll
+    for (int t = 0; t < AZW; t++) {
+        o.z[t] = (double)f[t];
+        nd |= (unsigned)in_interval(f[t], p.nd_lo, p.nd_hi) << t;
+    }
+    o.nd = nd;
+}
+
+// dL/dslope of row r, columns c0-1 .. c0+4 (zeros outside the raster).
+template <bool VEC>
+__device__ __forceinline__ void load_g(const AdjParams &p, int r, int c0, int lane, bool full, float gg[APW])
+{
+    float v[ACPT] = {0.f, 0.f, 0.f, 0.f};
+    float h = 0.f;
+    if (r >= 0 && r < p.rows) {
+        const float *rowp = p.g + (long long)r * p.pitch;
+        if (VEC && full) {
+            const float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
+            v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < ACPT; k++)
+                if (c0 + k < p.cols) v[k] = rowp[c0 + k];
+        }
+        const int a = lane == 0 ? c0 - 1 : (lane == 63 ? c0 + ACPT : c0);
+        h = (a >= 0 && a < p.cols) ? rowp[a] : 0.f;
+    }
+    const float l = lower_lane(v[3]), rr = upper_lane(v[0]);
+    gg[0] = lane == 0 ? h : l;
+#pragma unroll
+    for (int k = 0; k < ACPT; k++) gg[k + 1] = v[k];
+    gg[APW - 1] = lane == 63 ? h : rr;
+}
+
+// p of row k (columns c0-1 .. c0+4) from z rows k-1, k, k+1 and g row k.
+__device__ __forceinline__ void p_row(const AdjParams &p, int k, int c0, const ZRow &a, const ZRow &b, const ZRow &c,
+                                      const float gg[APW], float px[APW], float py[APW])
+{
+    const bool row_ok = k >= 1 && k <= p.rows - 2;
+    const unsigned ndw = a.nd | b.nd | c.nd;
+#pragma unroll
+    for (int t = 0; t < APW; t++) {
+        // z index t+1 is the column c0-1+t
+        const int col = c0 - 1 + t;
+        const bool ok = row_ok && col >= 1 && col <= p.cols - 2 && !((ndw >> t) & 7u);
+        const double nx = (__builtin_fma(2.0, b.z[t + 2], a.z[t + 2]) + c.z[t + 2]) -
+                          (__builtin_fma(2.0, b.z[t], a.z[t]) + c.z[t]);
+        const double ny = (__builtin_fma(2.0, a.z[t + 1], a.z[t]) + a.z[t + 2]) -
+                          (__builtin_fma(2.0, c.z[t + 1], c.z[t]) + c.z[t + 2]);
+        const double q = __builtin_fma(nx, nx, ny * ny);
+        const double f = (double)gg[t] * p.inv8cs * __builtin_amdgcn_rsq(q);
+        px[t] = (ok && q > 0) ? (float)(f * nx) : 0.f;
+        py[t] = (ok && q > 0) ? (float)(f * ny) : 0.f;
+    }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void slope_adjoint_kernel(AdjParams p)
+{
+    const int lane = threadIdx.x & 63;
+    const int c0 = (blockIdx.x * blockDim.x + threadIdx.x) * ACPT;
+    const int rbeg = blockIdx.y * p.rows_per_strip;
+    const int rend = min(rbeg + p.rows_per_strip, p.rows);
+    const int wave_c0 = __builtin_amdgcn_readfirstlane(c0);
+    const bool full = wave_c0 + 64 * ACPT <= p.cols;
+
+    // z rows r-2.. and p rows r-1, r for the first output row r = rbeg
+    ZRow zA, zB, zC;  // rows r, r+1 kept; third slot loads r+2
+    float pxm[APW], pym[APW], px0[APW], py0[APW], px1[APW], py1[APW], gg[APW];
+    {
+        ZRow zm2, zm1;
+        load_z<VEC>(p, rbeg - 2, c0, lane, full, zm2);
+        load_z<VEC>(p, rbeg - 1, c0, lane, full, zm1);
+        load_z<VEC>(p, rbeg, c0, lane, full, zA);
+        load_z<VEC>(p, rbeg + 1, c0, lane, full, zB);
+        load_g<VEC>(p, rbeg - 1, c0, lane, full, gg);
+        p_row(p, rbeg - 1, c0, zm2, zm1, zA, gg, pxm, pym);
+        load_g<VEC>(p, rbeg, c0, lane, full, gg);
+        p_row(p, rbeg, c0, zm1, zA, zB, gg, px0, py0);
+    }
+    for (int r = rbeg; r < rend; r++) {
+        load_z<VEC>(p, r + 2, c0, lane, full, zC);
+        load_g<VEC>(p, r + 1, c0, lane, full, gg);
+        p_row(p, r + 1, c0, zA, zB, zC, gg, px1, py1);
+        float out[ACPT];
+        float Px[APW];
+#pragma unroll
+        for (int t = 0; t < APW; t++) Px[t] = __builtin_fmaf(2.f, px0[t], pxm[t] + px1[t]);
+#pragma unroll
+        for (int j = 0; j < ACPT; j++) {
+            const float gx = Px[j] - Px[j + 2];
+            const float qdn = __builtin_fmaf(2.f, py1[j + 1], py1[j] + py1[j + 2]);
+            const float qup = __builtin_fmaf(2.f, pym[j + 1], pym[j] + pym[j + 2]);
+            out[j] = gx + (qdn - qup);
+        }
+        float *dst = p.gdem + (long long)r * p.pitch + c0;
+        if (VEC && full) {
+            *reinterpret_cast<float4 *>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < ACPT; j++)
+                if (c0 + j < p.cols) dst[j] = out[j];
+        }
+        // rotate
+        zA = zB;
+        zB = zC;
+#pragma unroll
+        for (int t = 0; t < APW; t++) {
+            pxm[t] = px0[t];
+            pym[t] = py0[t];
+            px0[t] = px1[t];
+            py0[t] = py1[t];
+        }
+    }
+}
+
+}  // namespace
+
+}  // namespace pargo
+
+using pargo::fail;
+
+extern "C" int pargo_slope_adjoint(pargo_ctx *ctx, const pargo_grid *g, const float *dem, const float *grad_slope,
+                                   float *grad_dem)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!g || !dem || !grad_slope || !grad_dem) return fail(ctx, PARGO_E_INVALID, "pargo_slope_adjoint: null argument");
+    if (g->rows < 3 || g->cols < 3)
+        return fail(ctx, PARGO_E_INVALID, "the dimensions (%d %d) are smaller than the 3x3 neighbourhood", g->rows,
+                    g->cols);
+    const long long pitch = g->pitch ? g->pitch : g->cols;
+    if (pitch < g->cols) return fail(ctx, PARGO_E_INVALID, "pitch %lld < cols %d", pitch, g->cols);
+    const float csf = (float)g->cellsize;
+    if (!(csf > 0) || !std::isfinite(csf)) return fail(ctx, PARGO_E_INVALID, "cell size must be > 0");
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    pargo::AdjParams p{};
+    p.dem = dem;
+    p.g = grad_slope;
+    p.gdem = grad_dem;
+    p.pitch = pitch;
+    p.rows = g->rows;
+    p.cols = g->cols;
+    p.rows_per_strip = 32;
+    pargo::nodata_interval(g->nodata, 0.0000001, &p.nd_lo, &p.nd_hi);
+    if (p.nd_lo > p.nd_hi) {
+        p.nd_lo = INFINITY;
+        p.nd_hi = -INFINITY;
+    }
+    p.inv8cs = 1.0 / (8.0 * (double)csf);
+    auto al = [](const void *q) { return ((uintptr_t)q % 16) == 0; };
+    const bool vec = pitch % 4 == 0 && al(dem) && al(grad_slope) && al(grad_dem);
+    dim3 block(256);
+    dim3 grid((g->cols + 256 * pargo::ACPT - 1) / (256 * pargo::ACPT), (g->rows + p.rows_per_strip - 1) / p.rows_per_strip);
+    if (vec) hipLaunchKernelGGL(pargo::slope_adjoint_kernel<true>, grid, block, 0, ctx->stream, p);
+    else hipLaunchKernelGGL(pargo::slope_adjoint_kernel<false>, grid, block, 0, ctx->stream, p);
+    return pargo::check_launch(ctx, "pargo_slope_adjoint");
+}

@@ -82,7 +82,8 @@ struct IdwParams {
     const int32_t *__restrict__ mask;
     float *__restrict__ out;
     unsigned long long *__restrict__ cand_out;
-    double grad_scale_unused;
+    const float *__restrict__ grad_out;  // adjoint: dL/d(out), rows x cols
+    double *__restrict__ grad_acc;       // adjoint: per bucket slot
     double minX, maxY, block, buffer, cs;
     int rows, cols, blockRows, blockCols, maxRing;
     int k, power;
@@ -204,7 +205,9 @@ __device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, doub
     return (tail + 1 < K) ? tail + 1 : K;
 }
 
-template <int MAXK>
+// ADJOINT: instead of writing the grid, add g(cell) * w_i / W to the
+// gradient of each selected sample (bucket order, FP64 atomics).
+template <int MAXK, bool ADJOINT>
 __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -237,13 +240,20 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                     wsum += w[i];
                 }
             }
-            double acc = 0.0;
+            if (ADJOINT) {
+                const double gc = (double)p.grad_out[(long long)row * p.cols + col];
 #pragma unroll
-            for (int i = 0; i < MAXK; i++)
-                if (i < num) acc += p.v[slot[i]] * w[i] / wsum;
-            result = (float)acc;
+                for (int i = 0; i < MAXK; i++)
+                    if (i < num && gc != 0.0) atomicAdd(&p.grad_acc[slot[i]], gc * w[i] / wsum);
+            } else {
+                double acc = 0.0;
+#pragma unroll
+                for (int i = 0; i < MAXK; i++)
+                    if (i < num) acc += p.v[slot[i]] * w[i] / wsum;
+                result = (float)acc;
+            }
         }
-        p.out[(long long)row * p.cols + col] = result;
+        if (!ADJOINT) p.out[(long long)row * p.cols + col] = result;
     }
     if (p.cand_out) {
         unsigned long long c = ncand;
@@ -252,23 +262,32 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
     }
 }
 
-template <int MAXK>
+template <int MAXK, bool ADJOINT>
 static void launch_idw_k(const IdwParams &p, hipStream_t s)
 {
     dim3 block(256);
     dim3 grid((p.cols + 15) / 16, (p.rows + 15) / 16);
-    hipLaunchKernelGGL((idw_kernel<MAXK>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((idw_kernel<MAXK, ADJOINT>), grid, block, 0, s, p);
 }
 
+template <bool ADJOINT>
 static int launch_idw(const IdwParams &p, hipStream_t s)
 {
-    if (p.k <= 4) launch_idw_k<4>(p, s);
-    else if (p.k <= 8) launch_idw_k<8>(p, s);
-    else if (p.k <= 12) launch_idw_k<12>(p, s);
-    else if (p.k <= 16) launch_idw_k<16>(p, s);
-    else if (p.k <= 32) launch_idw_k<32>(p, s);
+    if (p.k <= 4) launch_idw_k<4, ADJOINT>(p, s);
+    else if (p.k <= 8) launch_idw_k<8, ADJOINT>(p, s);
+    else if (p.k <= 12) launch_idw_k<12, ADJOINT>(p, s);
+    else if (p.k <= 16) launch_idw_k<16, ADJOINT>(p, s);
+    else if (p.k <= 32) launch_idw_k<32, ADJOINT>(p, s);
     else return PARGO_E_UNSUPPORTED;
     return PARGO_OK;
+}
+
+// grad_vals[file index] = grad_acc[bucket slot]
+__global__ void idw_unbucket_kernel(const double *__restrict__ acc, const int *__restrict__ perm, int n,
+                                    double *__restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[perm[q]] = acc[q];
 }
 
 int idw_fill_params(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
@@ -445,9 +464,33 @@ int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, 
     p.out = out;
     p.cand_out = candidates_out;
     if (candidates_out) PARGO_HIP(ctx, hipMemsetAsync(candidates_out, 0, sizeof(unsigned long long), ctx->stream));
-    rc = pargo::launch_idw(p, ctx->stream);
+    rc = pargo::launch_idw<false>(p, ctx->stream);
     if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
     return pargo::check_launch(ctx, "pargo_idw_run");
+}
+
+int pargo_idw_adjoint(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                      const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, const float *grad_out,
+                      double *grad_vals)
+{
+    pargo::IdwParams p;
+    int rc = pargo::idw_fill_params(ctx, plan, k, power, buffer, mask, mask_rows, mask_cols, mask_nodata, p);
+    if (rc) return rc;
+    if (!grad_out || !grad_vals) return fail(ctx, PARGO_E_INVALID, "pargo_idw_adjoint: null gradient pointer");
+    rc = pargo::bind(ctx);
+    if (rc) return rc;
+    rc = pargo::ensure_scratch(ctx, sizeof(double) * (size_t)plan->n);
+    if (rc) return rc;
+    p.grad_out = grad_out;
+    p.grad_acc = (double *)ctx->scratch;
+    PARGO_HIP(ctx, hipMemsetAsync(p.grad_acc, 0, sizeof(double) * (size_t)plan->n, ctx->stream));
+    rc = pargo::launch_idw<true>(p, ctx->stream);
+    if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
+    rc = pargo::check_launch(ctx, "pargo_idw_adjoint");
+    if (rc) return rc;
+    hipLaunchKernelGGL(pargo::idw_unbucket_kernel, dim3((plan->n + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double *)p.grad_acc, (const int *)plan->d_perm, plan->n, grad_vals);
+    return pargo::check_launch(ctx, "pargo_idw_adjoint (unbucket)");
 }
 
 }  // extern "C"
