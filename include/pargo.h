@@ -131,6 +131,36 @@ int pargo_slope_d8(pargo_ctx *ctx, const pargo_grid *grid, const float *dem,
                    int method, int unit, int dirtype, float *slope_out,
                    uint8_t *code_out, uint32_t *nodata_mask_out);
 
+/* ---- many rasters ---------------------------------------------------- */
+
+/* One raster of a batch: its descriptor and device buffers.  All tiles of a
+ * batch must agree on which outputs are NULL. */
+typedef struct {
+    pargo_grid grid;
+    const float *dem;
+    float *slope_out;
+    uint8_t *code_out;
+} pargo_tile;
+
+#define PARGO_BATCH_PER_TILE 0   /* one pargo_slope_d8 per tile, stream-ordered */
+#define PARGO_BATCH_PERSISTENT 1 /* one persistent launch over a device work queue */
+#define PARGO_BATCH_GRAPH 2      /* the persistent launch captured once in a HIP graph and replayed */
+
+/* Slope and/or D8 of every tile (`tiles` is a host array).  The reference
+ * spreads one raster over MPI ranks by row bands (DeComposition::rowDcmp,
+ * gpro/deComposition.h:148-182) and balances bands by estimated load
+ * (ComputeLayer, gpro/computeLayer.h:378-443); here (tile, column band, row
+ * strip) items of many rasters are handed to waves from one queue, largest
+ * tiles first.  Results are identical to per-tile pargo_slope_d8 calls. */
+int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                         int mode);
+
+/* Times the persistent launch for each candidate shape (columns per lane 2
+ * or 4, rows per strip 16, 32 or 64) on this batch, keeps the fastest for
+ * later batches on ctx and reports it (outputs may be NULL). */
+int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                         int *cols_per_lane, int *rows_per_strip);
+
 /* ---- IDW ------------------------------------------------------------ */
 
 /* Output grid of IDW for these samples (host arrays, file order), by the

@@ -51,6 +51,9 @@ _SIGS = {
     "pargo_slope_adjoint": (_c.c_int, [_P, _P, _P, _P, _P]),
     "pargo_idw_adjoint": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
                                      _c.c_int, _P, _P]),
+    "pargo_batch_slope_d8": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "pargo_batch_autotune": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_int),
+                                        _c.POINTER(_c.c_int)]),
     "pargo_synth_dem": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_int]),
     "pargo_synth_plateaus": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_double]),
     "pargo_synth_nodata_discs": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_double,

@@ -160,6 +160,7 @@ int pargo_destroy(pargo_ctx *ctx)
     if (ctx->own_stream) hipStreamSynchronize(ctx->own_stream);
     if (ctx->stream && ctx->stream != ctx->own_stream) hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) hipFree(ctx->scratch);
+    pargo::free_batch_plan(ctx);
     if (ctx->fix_mem) hipFree(ctx->fix_mem);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;

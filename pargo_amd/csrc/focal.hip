@@ -29,6 +29,9 @@
 // moderate range), the square root in FP32: relative error below 5e-7.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
 
 #include "focal_common.h"
 #include "internal.h"
@@ -84,8 +87,11 @@ struct FocalParams {
     // Cells whose window holds Inf or |z| >= 2^40 are queued here and
     // recomputed by focal_fixup_kernel with the reference-order FP64 code.
     unsigned *fix_count;  // [0] queued cells, [1] finished fixup blocks
-    unsigned *fix_list;   // linear cell indices r * cols + c
+    uint2 *fix_list;      // (tile index, linear cell index r * cols + c)
     unsigned fix_cap;
+    unsigned tile_id;     // index of this raster in a batch (0 otherwise)
+    int vec;              // 16-byte aligned rows: vector loads and stores
+    int waves_x;          // batch: wave-wide column bands of this tile
 };
 
 template <int CPT>
@@ -386,7 +392,7 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
             // queue the cell for the exact pass
             if (((spw >> j) & 7u) && !((!DO_SLOPE || nds) && (!DO_D8 || n8))) {
                 const unsigned k = atomicAdd(p.fix_count, 1u);
-                if (k < p.fix_cap) p.fix_list[k] = (unsigned)r * (unsigned)p.cols + (unsigned)(c0 + j);
+                if (k < p.fix_cap) p.fix_list[k] = make_uint2(p.tile_id, (unsigned)r * (unsigned)p.cols + (unsigned)(c0 + j));
             }
         }
     }
@@ -468,15 +474,12 @@ __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
     }
 }
 
+// One wave computes output rows [rbeg, rend) of the 64*CPT columns that
+// start at wave_c0 (wave-uniform).
 template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
-__global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
+__device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
 {
-    const int lane = threadIdx.x & 63;
-    const int c0 = (blockIdx.x * blockDim.x + threadIdx.x) * CPT;
-    const int rbeg = blockIdx.y * p.rows_per_strip;
-    const int rend = min(rbeg + p.rows_per_strip, p.rows);
-    // first column of the wave, read from lane 0 so that it is wave-uniform
-    const int wave_c0 = __builtin_amdgcn_readfirstlane(c0);
+    const int c0 = wave_c0 + lane * CPT;
     const bool full = wave_c0 + 64 * CPT <= p.cols;
     unsigned colb = 0;  // bit j: column c0+j is on (or past) the border
 #pragma unroll
@@ -505,16 +508,66 @@ __global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
     }
 }
 
-// Recomputes queued cells with the reference-order FP64 code.  If the queue
-// overflowed it scans the whole grid for special windows instead.  The last
-// block to finish resets the queue for the next call on the stream.
-template <bool DO_SLOPE, bool DO_D8>
-__global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams p)
+// One raster: block = 4 waves side by side, one row strip per block.
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+__global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
 {
-    const unsigned n = __atomic_load_n(&p.fix_count[0], __ATOMIC_RELAXED);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave_c0 = (blockIdx.x * 4 + wave) * 64 * CPT;
+    const int rbeg = blockIdx.y * p.rows_per_strip;
+    const int rend = min(rbeg + p.rows_per_strip, p.rows);
+    focal_strip<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
+}
+
+// Many rasters in one persistent launch: every wave takes (tile, column
+// band, row strip) items from a device queue until it is empty.  Items are
+// numbered tile by tile (first[t] = first item of tile t; tiles are in
+// decreasing size so that the tail of the queue holds the small items).
+template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+__global__ __launch_bounds__(256) void focal_batch_kernel(const FocalParams *__restrict__ tiles,
+                                                          const unsigned *__restrict__ first, int ntiles,
+                                                          unsigned *__restrict__ queue, unsigned total)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned item = 0;
+        if (lane == 0) item = atomicAdd(queue, 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= total) break;
+        int lo = 0, hi = ntiles - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const FocalParams p = tiles[lo];
+        const unsigned local = item - first[lo];
+        const int wx = (int)(local % (unsigned)p.waves_x), strip = (int)(local / (unsigned)p.waves_x);
+        const int rbeg = strip * p.rows_per_strip;
+        const int rend = min(rbeg + p.rows_per_strip, p.rows);
+        if (p.vec) focal_strip<CPT, true, DO_SLOPE, DO_D8, HORN>(p, wx * 64 * CPT, lane, rbeg, rend);
+        else focal_strip<CPT, false, DO_SLOPE, DO_D8, HORN>(p, wx * 64 * CPT, lane, rbeg, rend);
+    }
+}
+
+// Zeroes the batch queue head (a kernel rather than a memset, so that the
+// sequence can be captured in a HIP graph and replayed).
+__global__ void focal_queue_reset(unsigned *queue) { queue[0] = 0; }
+
+// Recomputes queued cells with the reference-order FP64 code.  If the queue
+// overflowed it scans every raster for special windows instead.  The last
+// block to finish resets the queue for the next call on the stream.
+// `tiles` is NULL for a single raster (then `one` describes it).
+template <bool DO_SLOPE, bool DO_D8>
+__global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams one, const FocalParams *__restrict__ tiles,
+                                                          int ntiles)
+{
+    const unsigned *cnt = one.fix_count;
+    const unsigned n = __atomic_load_n(&cnt[0], __ATOMIC_RELAXED);
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    auto fix = [&](int r, int c, bool check) {
+    auto fix = [&](const FocalParams &p, int r, int c, bool check) {
         double d[9];
         bool nd = false, nd8 = false, sp = false;
         for (int k = 0; k < 9; k++) {
@@ -533,23 +586,28 @@ __global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams p)
         }
         if (DO_SLOPE && !nd) p.slope[o] = slope_unit((float)slope_generic(d, p.method, p.cs), p.unit);
     };
-    if (n <= p.fix_cap) {
+    if (n <= one.fix_cap) {
         for (unsigned k = tid; k < n; k += stride) {
-            const unsigned cell = p.fix_list[k];
-            fix((int)(cell / (unsigned)p.cols), (int)(cell % (unsigned)p.cols), false);
+            const uint2 e = one.fix_list[k];
+            const FocalParams &p = tiles ? tiles[e.x] : one;
+            fix(p, (int)(e.y / (unsigned)p.cols), (int)(e.y % (unsigned)p.cols), false);
         }
     } else {
-        const long long cells = (long long)(p.rows - 2) * (p.cols - 2);
-        for (long long k = tid; k < cells; k += stride)
-            fix(1 + (int)(k / (p.cols - 2)), 1 + (int)(k % (p.cols - 2)), true);
+        for (int t = 0; t < (tiles ? ntiles : 1); t++) {
+            const FocalParams &p = tiles ? tiles[t] : one;
+            const long long cells = (long long)(p.rows - 2) * (p.cols - 2);
+            for (long long k = tid; k < cells; k += stride)
+                fix(p, 1 + (int)(k / (p.cols - 2)), 1 + (int)(k % (p.cols - 2)), true);
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        const unsigned done = atomicAdd(&p.fix_count[1], 1u);
+        unsigned *c = one.fix_count;
+        const unsigned done = atomicAdd(&c[1], 1u);
         if (done == gridDim.x - 1) {
-            p.fix_count[0] = 0;
-            p.fix_count[1] = 0;
+            c[0] = 0;
+            c[1] = 0;
             __threadfence();
         }
     }
@@ -579,12 +637,13 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
     }
 }
 
-static void launch_fixup(const FocalParams &p, bool do_slope, bool do_d8, hipStream_t s)
+static void launch_fixup(const FocalParams &p, const FocalParams *tiles, int ntiles, bool do_slope, bool do_d8,
+                         hipStream_t s)
 {
     dim3 block(256), grid(kFixupBlocks);
-    if (do_slope && do_d8) hipLaunchKernelGGL((focal_fixup_kernel<true, true>), grid, block, 0, s, p);
-    else if (do_slope) hipLaunchKernelGGL((focal_fixup_kernel<true, false>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p);
+    if (do_slope && do_d8) hipLaunchKernelGGL((focal_fixup_kernel<true, true>), grid, block, 0, s, p, tiles, ntiles);
+    else if (do_slope) hipLaunchKernelGGL((focal_fixup_kernel<true, false>), grid, block, 0, s, p, tiles, ntiles);
+    else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p, tiles, ntiles);
 }
 
 static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
@@ -626,7 +685,7 @@ static LaunchCfg default_cfg(int rows)
 static int ensure_fixup(pargo_ctx *ctx)
 {
     if (ctx->fix_mem) return PARGO_OK;
-    const size_t bytes = (64 + (size_t)kFixupCap) * sizeof(unsigned);
+    const size_t bytes = 64 * sizeof(unsigned) + (size_t)kFixupCap * sizeof(uint2);
     hipError_t e = hipMalloc(&ctx->fix_mem, bytes);
     if (e != hipSuccess) return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the fixup queue: %s", bytes,
                                      hipGetErrorString(e));
@@ -634,6 +693,13 @@ static int ensure_fixup(pargo_ctx *ctx)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(ctx, PARGO_E_DEVICE, "clearing the fixup queue: %s", hipGetErrorString(e));
     return PARGO_OK;
+}
+
+static void attach_fixup(pargo_ctx *ctx, FocalParams &p)
+{
+    p.fix_count = (unsigned *)ctx->fix_mem;
+    p.fix_list = (uint2 *)((unsigned *)ctx->fix_mem + 64);
+    p.fix_cap = kFixupCap;
 }
 
 // Validates the request and fills the kernel parameters.
@@ -687,6 +753,7 @@ static int prepare(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int me
     auto aligned = [](const void *ptr, size_t a) { return ((uintptr_t)ptr % a) == 0; };
     vec = (pitch % 4 == 0) && aligned(dem, 16) && (!slope_out || aligned(slope_out, 16)) &&
           (!code_out || aligned(code_out, 4));
+    p.vec = vec ? 1 : 0;
     return PARGO_OK;
 }
 
@@ -707,14 +774,219 @@ static int run_focal(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int 
     if (!do_slope && !do_d8) return fail(ctx, PARGO_E_INVALID, "%s: a NoData mask needs a slope or code output", name);
     rc = ensure_fixup(ctx);
     if (rc) return rc;
-    p.fix_count = (unsigned *)ctx->fix_mem;
-    p.fix_list = (unsigned *)ctx->fix_mem + 64;
-    p.fix_cap = kFixupCap;
+    attach_fixup(ctx, p);
     launch_focal(p, cfg, vec, do_slope, do_d8, ctx->stream);
     rc = check_launch(ctx, name);
     if (rc) return rc;
-    launch_fixup(p, do_slope, do_d8, ctx->stream);
+    launch_fixup(p, nullptr, 1, do_slope, do_d8, ctx->stream);
     return check_launch(ctx, name);
+}
+
+// ---- batches ------------------------------------------------------------
+
+struct BatchPlan {
+    std::vector<unsigned char> key;  // tiles + parameters the plan was built for
+    void *d_mem = nullptr;
+    FocalParams *d_tiles = nullptr;
+    unsigned *d_first = nullptr;
+    unsigned *d_queue = nullptr;
+    std::vector<FocalParams> h_tiles;  // original order, for per-tile launches
+    unsigned total = 0;
+    int ntiles = 0;
+    int cpt = 4;
+    int blocks = 0;
+    bool do_slope = false, do_d8 = false, horn = false;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+void free_batch_plan(pargo_ctx *ctx)
+{
+    BatchPlan *b = ctx->batch;
+    if (!b) return;
+    hipStreamSynchronize(ctx->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_mem) hipFree(b->d_mem);
+    delete b;
+    ctx->batch = nullptr;
+}
+
+template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+static int batch_blocks(pargo_ctx *ctx)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>, 256, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 2;
+    const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
+    return per_cu * cus;
+}
+
+template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+static void launch_batch_t(const BatchPlan &b, hipStream_t s)
+{
+    hipLaunchKernelGGL((focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>), dim3(b.blocks), dim3(256), 0, s,
+                       (const FocalParams *)b.d_tiles, (const unsigned *)b.d_first, b.ntiles, b.d_queue, b.total);
+}
+
+template <int CPT>
+static void launch_batch_c(const BatchPlan &b, hipStream_t s)
+{
+    if (b.do_slope && b.do_d8) {
+        if (b.horn) launch_batch_t<CPT, true, true, true>(b, s);
+        else launch_batch_t<CPT, true, true, false>(b, s);
+    } else if (b.do_slope) {
+        if (b.horn) launch_batch_t<CPT, true, false, true>(b, s);
+        else launch_batch_t<CPT, true, false, false>(b, s);
+    } else {
+        launch_batch_t<CPT, false, true, true>(b, s);
+    }
+}
+
+template <int CPT>
+static int blocks_c(pargo_ctx *ctx, const BatchPlan &b)
+{
+    if (b.do_slope && b.do_d8) return b.horn ? batch_blocks<CPT, true, true, true>(ctx) : batch_blocks<CPT, true, true, false>(ctx);
+    if (b.do_slope) return b.horn ? batch_blocks<CPT, true, false, true>(ctx) : batch_blocks<CPT, true, false, false>(ctx);
+    return batch_blocks<CPT, false, true, true>(ctx);
+}
+
+// The persistent sequence: queue reset, batch kernel, fixup.
+static void enqueue_batch(const BatchPlan &b, hipStream_t s)
+{
+    hipLaunchKernelGGL(focal_queue_reset, dim3(1), dim3(1), 0, s, b.d_queue);
+    if (b.cpt == 2) launch_batch_c<2>(b, s);
+    else launch_batch_c<4>(b, s);
+    launch_fixup(b.h_tiles[0], b.d_tiles, b.ntiles, b.do_slope, b.do_d8, s);
+}
+
+static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype, int cpt,
+                       int rows_per_strip, BatchPlan **out)
+{
+    if (!tiles || ntiles < 1) return fail(ctx, PARGO_E_INVALID, "empty batch");
+    if (ntiles > (1 << 24)) return fail(ctx, PARGO_E_INVALID, "too many tiles");
+    std::vector<unsigned char> key(sizeof(pargo_tile) * (size_t)ntiles + 6 * sizeof(int));
+    std::memcpy(key.data(), tiles, sizeof(pargo_tile) * (size_t)ntiles);
+    const int extra[6] = {ntiles, method, unit, dirtype, cpt, rows_per_strip};
+    std::memcpy(key.data() + sizeof(pargo_tile) * (size_t)ntiles, extra, sizeof(extra));
+    if (ctx->batch && ctx->batch->key == key) {
+        *out = ctx->batch;
+        return PARGO_OK;
+    }
+    free_batch_plan(ctx);
+    int rc = ensure_fixup(ctx);
+    if (rc) return rc;
+    BatchPlan *b = new (std::nothrow) BatchPlan();
+    if (!b) return fail(ctx, PARGO_E_NOMEM, "out of host memory");
+    b->key = std::move(key);
+    b->cpt = cpt;
+    b->ntiles = ntiles;
+    b->h_tiles.resize(ntiles);
+    const bool do_slope = tiles[0].slope_out != nullptr, do_d8 = tiles[0].code_out != nullptr;
+    if (!do_slope && !do_d8) {
+        delete b;
+        return fail(ctx, PARGO_E_INVALID, "batch tiles have no outputs");
+    }
+    for (int t = 0; t < ntiles; t++) {
+        if ((tiles[t].slope_out != nullptr) != do_slope || (tiles[t].code_out != nullptr) != do_d8) {
+            delete b;
+            return fail(ctx, PARGO_E_INVALID, "tile %d: all tiles must request the same outputs", t);
+        }
+        bool vec = false;
+        rc = prepare(ctx, &tiles[t].grid, tiles[t].dem, method, unit, dirtype, tiles[t].slope_out, tiles[t].code_out,
+                     nullptr, b->h_tiles[t], vec);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        attach_fixup(ctx, b->h_tiles[t]);
+        b->h_tiles[t].rows_per_strip = rows_per_strip;
+        b->h_tiles[t].waves_x = (tiles[t].grid.cols + 64 * cpt - 1) / (64 * cpt);
+    }
+    b->do_slope = do_slope;
+    b->do_d8 = do_d8;
+    b->horn = method == PARGO_SLOPE_FD;
+    for (int t = 0; t < ntiles; t++)
+        if (!(b->h_tiles[t].cs >= 0x1p-20 && b->h_tiles[t].cs <= 0x1p40)) b->horn = false;
+    // device order: largest tiles first
+    std::vector<int> order(ntiles);
+    for (int t = 0; t < ntiles; t++) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return (long long)tiles[a].grid.rows * tiles[a].grid.cols > (long long)tiles[c].grid.rows * tiles[c].grid.cols;
+    });
+    std::vector<FocalParams> dev(ntiles);
+    std::vector<unsigned> first(ntiles);
+    unsigned long long total = 0;
+    for (int i = 0; i < ntiles; i++) {
+        dev[i] = b->h_tiles[order[i]];
+        dev[i].tile_id = (unsigned)i;
+        first[i] = (unsigned)total;
+        const int strips = (dev[i].rows + rows_per_strip - 1) / rows_per_strip;
+        total += (unsigned long long)strips * (unsigned long long)dev[i].waves_x;
+    }
+    if (total >= (1ull << 32)) {
+        delete b;
+        return fail(ctx, PARGO_E_INVALID, "batch has too many work items");
+    }
+    b->total = (unsigned)total;
+    const size_t bt = sizeof(FocalParams) * ntiles, bf = sizeof(unsigned) * ntiles;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    hipError_t e = hipMalloc(&b->d_mem, al(bt) + al(bf) + 256);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc for the batch plan: %s", hipGetErrorString(e));
+    }
+    b->d_tiles = (FocalParams *)b->d_mem;
+    b->d_first = (unsigned *)((char *)b->d_mem + al(bt));
+    b->d_queue = (unsigned *)((char *)b->d_mem + al(bt) + al(bf));
+    e = hipMemcpyAsync(b->d_tiles, dev.data(), bt, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->d_first, first.data(), bf, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(b->d_mem);
+        delete b;
+        return fail(ctx, PARGO_E_DEVICE, "uploading the batch plan: %s", hipGetErrorString(e));
+    }
+    b->blocks = cpt == 2 ? blocks_c<2>(ctx, *b) : blocks_c<4>(ctx, *b);
+    ctx->batch = b;
+    *out = b;
+    return PARGO_OK;
+}
+
+static int run_batch(pargo_ctx *ctx, BatchPlan *b, int mode, int rows_per_strip)
+{
+    const hipStream_t s = ctx->stream;
+    if (mode == PARGO_BATCH_PER_TILE) {
+        LaunchCfg cfg{b->cpt, rows_per_strip};
+        for (int t = 0; t < b->ntiles; t++) {
+            const FocalParams &p = b->h_tiles[t];
+            launch_focal(p, cfg, p.vec != 0, b->do_slope, b->do_d8, s);
+            launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, s);
+        }
+        return check_launch(ctx, "pargo_batch_slope_d8 (per tile)");
+    }
+    if (mode == PARGO_BATCH_PERSISTENT) {
+        enqueue_batch(*b, s);
+        return check_launch(ctx, "pargo_batch_slope_d8 (persistent)");
+    }
+    if (!b->exec) {
+        // capture on a private stream so that a caller's legacy default
+        // stream can be used for everything else
+        hipStream_t cs;
+        PARGO_HIP(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            enqueue_batch(*b, cs);
+            e = hipStreamEndCapture(cs, &b->graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        hipStreamDestroy(cs);
+        if (e != hipSuccess) return fail(ctx, PARGO_E_DEVICE, "capturing the batch graph: %s", hipGetErrorString(e));
+    }
+    PARGO_HIP(ctx, hipGraphLaunch(b->exec, s));
+    return PARGO_OK;
 }
 
 }  // namespace pargo
@@ -741,6 +1013,62 @@ int pargo_slope_d8(pargo_ctx *ctx, const pargo_grid *grid, const float *dem, int
 {
     return pargo::run_focal(ctx, grid, dem, method, unit, dirtype, slope_out, code_out, nodata_mask_out,
                             "pargo_slope_d8");
+}
+
+int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                         int mode)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (mode < PARGO_BATCH_PER_TILE || mode > PARGO_BATCH_GRAPH)
+        return pargo::fail(ctx, PARGO_E_INVALID, "unknown batch mode %d", mode);
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    pargo::BatchPlan *b = nullptr;
+    rc = pargo::build_batch(ctx, tiles, ntiles, method, unit, dirtype, ctx->batch_cpt, ctx->batch_rows, &b);
+    if (rc) return rc;
+    return pargo::run_batch(ctx, b, mode, ctx->batch_rows);
+}
+
+int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                         int *cols_per_lane, int *rows_per_strip)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    const int cpts[2] = {2, 4}, rps[3] = {16, 32, 64};
+    float best = INFINITY;
+    int bc = ctx->batch_cpt, br = ctx->batch_rows;
+    hipEvent_t e0, e1;
+    PARGO_HIP(ctx, hipEventCreate(&e0));
+    PARGO_HIP(ctx, hipEventCreate(&e1));
+    for (int c : cpts)
+        for (int r : rps) {
+            pargo::BatchPlan *b = nullptr;
+            rc = pargo::build_batch(ctx, tiles, ntiles, method, unit, dirtype, c, r, &b);
+            if (rc) break;
+            rc = pargo::run_batch(ctx, b, PARGO_BATCH_PERSISTENT, r);  // warm-up
+            if (rc) break;
+            hipEventRecord(e0, ctx->stream);
+            for (int k = 0; k < 2; k++) pargo::run_batch(ctx, b, PARGO_BATCH_PERSISTENT, r);
+            hipEventRecord(e1, ctx->stream);
+            hipEventSynchronize(e1);
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                bc = c;
+                br = r;
+            }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    ctx->batch_cpt = bc;
+    ctx->batch_rows = br;
+    ctx->batch_tuned = 1;
+    if (cols_per_lane) *cols_per_lane = bc;
+    if (rows_per_strip) *rows_per_strip = br;
+    return PARGO_OK;
 }
 
 }  // extern "C"

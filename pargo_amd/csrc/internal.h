@@ -10,6 +10,10 @@
 
 #include "../../include/pargo.h"
 
+namespace pargo {
+struct BatchPlan;
+}
+
 struct pargo_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -20,7 +24,17 @@ struct pargo_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     void *fix_mem = nullptr;  // focal fixup queue (counters + cell list)
+    // batched slope/D8: the last plan (with its captured graph) and the
+    // launch shape chosen by pargo_batch_autotune
+    pargo::BatchPlan *batch = nullptr;
+    int batch_cpt = 4;
+    int batch_rows = 32;
+    int batch_tuned = 0;
 };
+
+namespace pargo {
+void free_batch_plan(pargo_ctx *ctx);
+}
 
 namespace pargo {
 
