@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the focal-operator hot path on one MI355X.
 
-Workload (default): Horn slope + D8 flow direction (ESRI codes, dirType 256)
-fused in one pass over a 16384 x 16384 float32 DEM with 30 m cells: seeded
-fractal value-noise terrain, 2% plateaus, 1% NoData discs.  One step is one
+Headline workload: Horn slope + D8 flow direction (ESRI codes, dirType 256)
+fused in one pass over a 16384 x 16384 float32 DEM with 30 m cells (seeded
+fractal value-noise terrain, 2% plateaus, 1% NoData discs).  One step is one
 pargo_slope_d8 call over the whole DEM, input already resident in HBM.
 
-Prints one JSON line with Gcells/s, the HBM roofline fraction of the kernel
-(9 algorithmic bytes per cell: 4 B DEM read, 4 B slope and 1 B code written)
-and the CPU baseline (the FP64 restatement with one OpenMP row band per
-thread, slope sweep + D8 sweep, on a band of rows of the same DEM).
+The JSON line carries Gcells/s, the HBM roofline of the fused kernel (9
+algorithmic bytes per cell: 4 B DEM read, 4 B slope and 1 B code written)
+and the CPU baseline (the FP64 restatement, one OpenMP row band per thread,
+slope sweep + D8 sweep, on a band of rows of the same DEM).  Unless
+--no-secondary is given it also measures, under "secondary":
+  adjoint  slope adjoint vs forward on an 8192^2 DEM (time ratio)
+  idw      IDW of 20000 samples onto 4096^2 cells (K=12, p=2, 5 km)
+  batch    512 uneven tiles (256..4096 cells a side) per-tile vs one graph
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
@@ -17,6 +21,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -25,9 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 BYTES_PER_CELL = 9
 NODATA = -9999.0
 CELL = 30.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "focal_slope_d8_16384_hbm_traffic.txt")
 
 
 def log(*a):
@@ -41,24 +48,146 @@ def cpu_threads():
         return 16
 
 
+def event_time(torch, fn, n, warm=2):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(n):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / n
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if present."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def copy_probe():
+    """A plain 1 GiB device copy (known traffic: 1 GiB read + 1 GiB write),
+    the calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ms = event_time(torch, lambda: b.copy_(a), 5)
+    print(json.dumps({"copy_probe_bytes": 2 * 4 * n, "ms": ms, "GBs": 8 * n / ms / 1e6}), flush=True)
+
+
+def secondary_adjoint(torch, _lib, focal, synth):
+    N = 8192
+    z = synth.dem(synth.BASE_SEED + 3, N, N)
+    synth.add_nodata_discs(synth.BASE_SEED + 3, z, fraction=0.01)
+    dem = torch.from_numpy(z).cuda()
+    del z
+    s = torch.empty_like(dem)
+    g = torch.rand_like(dem)
+    gd = torch.empty_like(dem)
+    ctx = _lib.default_context(0)
+    ctx.bind_stream()
+    L = _lib.lib()
+    grid = focal.grid_of(dem, CELL, NODATA)
+    fwd = event_time(torch, lambda: ctx.check(L.pargo_slope(ctx.h, ctypes.byref(grid), _lib._ptr(dem), 0, 0,
+                                                            _lib._ptr(s), None)), 20)
+    adj = event_time(torch, lambda: ctx.check(L.pargo_slope_adjoint(ctx.h, ctypes.byref(grid), _lib._ptr(dem),
+                                                                    _lib._ptr(g), _lib._ptr(gd))), 20)
+    out = {"config": "Horn slope and its adjoint on an 8192^2 float32 DEM (1% NoData)",
+           "fwd_ms": round(fwd, 5), "adjoint_ms": round(adj, 5), "adjoint_fwd_ratio": round(adj / fwd, 3),
+           "fwd_GBs": round(8 * N * N / fwd / 1e6, 1), "adjoint_GBs": round(12 * N * N / adj / 1e6, 1),
+           "adjoint_bytes_per_cell": 12}
+    del dem, s, g, gd
+    torch.cuda.empty_cache()
+    return out
+
+
+def secondary_idw(torch, synth, IDWOperator, with_cpu):
+    xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 20000)
+    op = IDWOperator(CELL, 12, 2, 5000.0, 5000.0).read_samples(xs, ys, vs)
+    out, cand = op.run(count_candidates=True)
+    torch.cuda.synchronize()
+    c = int(cand.item())
+    ms = event_time(torch, lambda: op.run(out=out), 5, warm=1)
+    cells = op.rows * op.cols
+    flop = 5.0 * c + 49.0 * cells
+    res = {"config": "IDW of 20000 samples onto 4096^2 cells, K=12, p=2, searchRange 5000, blockSize 5000",
+           "ms": round(ms, 4), "Gcells_s": round(cells / ms / 1e6, 3), "candidates_per_cell": round(c / cells, 2),
+           "roofline": {"bound": "fp64 valu", "achieved": round(flop / ms / 1e9, 3), "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(flop / ms / 1e9 / FP64_PEAK_TFLOPS, 4),
+                        "flop_per_launch": flop}}
+    if with_cpu:
+        from oracle import pyoracle
+        import numpy as np
+        # bounded sample: the oracle computes rows 0..63 only; a (rows x 1)
+        # mask layer with 0 below row 63 makes every other cell a masked
+        # cell, which the reference skips before searching
+        nth = cpu_threads()
+        band_rows = 64
+        mask = np.zeros((op.rows, 1), dtype=np.int32)
+        mask[:band_rows, 0] = 1
+        t0 = time.perf_counter()
+        pyoracle.idw(xs, ys, vs, CELL, k=12, power=2, buffer=5000.0, block_size=5000.0, mask=mask,
+                     mask_nodata=-1, nthreads=nth)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(band_rows * op.cols / dt / 1e9, 5), "unit": "Gcells/s", "cores": nth,
+                               "kind": "port", "sample": "rows 0..%d of the grid (other rows masked out)" % (band_rows - 1)}
+    op.free()
+    return res
+
+
+def secondary_batch(torch, batch):
+    shapes = batch.tile_shapes(512, batch.synth.BASE_SEED + 1000)
+    dems = [torch.from_numpy(batch.tile_dem(t, r, c)).cuda() for t, (r, c) in enumerate(shapes)]
+    cells = sum(r * c for r, c in shapes)
+    b = batch.Batch(dems)
+    per = event_time(torch, lambda: b.run(batch.PER_TILE), 3, warm=1)
+    cpt, rows = b.autotune()
+    gr = event_time(torch, lambda: b.run(batch.GRAPH), 5, warm=1)
+    res = {"config": "512 tiles, side exp(U[ln 256, ln 4096]), aspect U[0.5, 2], 1-5% NoData, slope+D8",
+           "tiles": 512, "Gcells": round(cells / 1e9, 4), "per_tile_ms": round(per, 3), "graph_ms": round(gr, 3),
+           "speedup": round(per / gr, 2), "graph_Gcells_s": round(cells / gr / 1e6, 1),
+           "graph_GBs": round(9 * cells / gr / 1e6, 1), "autotuned": {"cols_per_lane": cpt, "rows_per_strip": rows}}
+    del b, dems
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=16384)
-    ap.add_argument("--cpu-rows", type=int, default=2048, help="rows of the DEM in the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=4096, help="rows of the DEM in the CPU baseline sample")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy used to calibrate counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
+        return
 
     import numpy as np
     import torch
-    from pargo_amd import _lib, focal, synth
+    from pargo_amd import _lib, batch, focal, synth
+    from pargo_amd.idw import IDWOperator
 
     t0 = time.time()
     z = synth.workload_dem("slope_d8", a.size, a.size)
@@ -85,7 +214,8 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    # Timed region: K steps, events around every launch on the launch stream.
+    # Timed region: K steps, HIP events around every launch on the stream the
+    # kernels run on (torch's current stream, which the context is bound to).
     stream = torch.cuda.current_stream(dev)
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
     torch.cuda.synchronize()
@@ -102,8 +232,7 @@ def main():
     cells = a.size * a.size
     value = cells / (ms_per_step / 1000.0) / 1e9
     achieved = BYTES_PER_CELL * cells / (avg_ms / 1000.0) / 1e9
-    log("kernel avg %.4f ms (min %.4f max %.4f), wall %.4f ms/step" % (avg_ms, min(kern_ms), max(kern_ms),
-                                                                      ms_per_step))
+    log("step avg %.4f ms (min %.4f max %.4f), wall %.4f ms/step" % (avg_ms, min(kern_ms), max(kern_ms), ms_per_step))
 
     out = {
         "metric": "Gcells/s slope+D8 on 16384^2 DEM, % HBM roofline; adjoint/fwd time ratio",
@@ -116,17 +245,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 in/out, f64 D8 decisions",
+        "dtype": "f64",
         "data": "synthetic (seeded fractal DEM, 2%% plateaus, %.2f%% NoData)" % (100 * nd_frac),
-        "config": {"workload": "slope+D8 fused, Horn FD, dirType 256", "rows": a.size, "cols": a.size,
-                   "cellsize_m": CELL, "dem_dtype": "float32", "outputs": "slope f32 + D8 code u8"},
+        "config": {"workload": "slope+D8 fused (Horn FD, tan units; D8 dirType 256) on a 16384x16384 float32 DEM",
+                   "rows": a.size, "cols": a.size, "cellsize_m": CELL,
+                   "io": "f32 DEM in, f32 slope + u8 D8 code out; D8 decisions in f64"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "focal_kernel<4,VEC,slope,d8,horn>", "kernel_ms": round(avg_ms, 5),
-                     "algorithmic_bytes_per_cell": BYTES_PER_CELL},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "kernel": "focal_kernel<4,true,true,true,true> + focal_fixup_kernel (empty queue)",
+                     "step_ms": round(avg_ms, 5), "algorithmic_bytes_per_cell": BYTES_PER_CELL},
     }
 
-    # Parity spot check and CPU baseline on a band of rows of the same DEM.
     if not a.no_cpu_baseline:
         from oracle import pyoracle
         rows_s = min(a.cpu_rows, a.size)
@@ -140,11 +269,18 @@ def main():
             times.append(time.perf_counter() - c0)
         tmed = statistics.median(times)
         ncell = rows_s * a.size
+        one = band[:512]
+        c0 = time.perf_counter()
+        pyoracle.slope(one, CELL, NODATA, "FD", nthreads=1)
+        pyoracle.d8(one, NODATA, 256, nthreads=1)
+        t1 = time.perf_counter() - c0
         out["cpu_baseline"] = {"value": round(ncell / tmed / 1e9, 4), "unit": "Gcells/s", "cores": nth,
                                "kind": "port",
-                               "sample": "rows 0..%d of the same DEM (%d cells), slope sweep + D8 sweep, FP64, "
-                                         "median of %d" % (rows_s - 1, ncell, a.cpu_repeats)}
-        # compare interior rows of the band (the band's last row is a border there)
+                               "sample": "rows 0..%d of the same DEM (%d cells), slope sweep + D8 sweep in FP64, "
+                                         "one row band per OpenMP thread, median of %d" % (rows_s - 1, ncell,
+                                                                                           a.cpu_repeats),
+                               "one_thread_Gcells_s": round(512 * a.size / t1 / 1e9, 4)}
+        # parity on the band's interior rows (its last row is a border there)
         gs = slope[: rows_s - 1].cpu().numpy()
         gc = code[: rows_s - 1].cpu().numpy().astype(np.int64)
         rs, rd = ref_s[: rows_s - 1], ref_d[: rows_s - 1]
@@ -156,6 +292,21 @@ def main():
         out["parity"] = {"d8_bitexact": d8_ok, "slope_max_rel": float(rel.max()),
                          "slope_rms_rel": float(np.sqrt(np.mean(rel ** 2))), "cells_checked": int(v.size)}
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        del ref_s, ref_d, band
+    del dem, slope, code, z
+    torch.cuda.empty_cache()
+
+    if not a.no_secondary:
+        sec = {}
+        t0 = time.time()
+        sec["adjoint"] = secondary_adjoint(torch, _lib, focal, synth)
+        log("adjoint done (%.1fs)" % (time.time() - t0))
+        sec["idw"] = secondary_idw(torch, synth, IDWOperator, not a.no_cpu_baseline)
+        log("idw done (%.1fs)" % (time.time() - t0))
+        sec["batch"] = secondary_batch(torch, batch)
+        log("batch done (%.1fs)" % (time.time() - t0))
+        out["secondary"] = sec
+        out["adjoint_fwd_ratio"] = sec["adjoint"]["adjoint_fwd_ratio"]
     print(json.dumps(out), flush=True)
 
 
