@@ -167,6 +167,10 @@ int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, in
  * extent rule of IDWOperator::readSamples (idwOperator.cpp:126-141). */
 int pargo_idw_grid(const double *xs, const double *ys, int n, float cellsize,
                    int *rows, int *cols, double *min_x, double *max_y);
+/* The same rule with the whole extent: extent = {minX, maxX, minY, maxY}
+ * after the reference's half-cell shift and rounding up to whole cells. */
+int pargo_idw_extent(const double *xs, const double *ys, int n, float cellsize,
+                     double extent[4], int *rows, int *cols);
 
 typedef struct pargo_idw_plan pargo_idw_plan;
 
@@ -192,6 +196,33 @@ int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, 
                   const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
                   float *out, unsigned long long *candidates_out);
 
+/* ---- load-balanced decomposition ------------------------------------ */
+
+/* Estimated compute load of IDW per cell of a coarse grid of
+ * ceil(rows/granularity) x ceil(cols/granularity) cells, cell size
+ * (float)(cellsize * granularity): IdwTransformation::Operator
+ * (apps/spatial/idw/idwTransformation.cpp:3-62) over ComputeLayer
+ * (gpro/computeLayer.h:208-260).  load = blocks visited + 2 * samples
+ * counted by a ring walk that visits only the full top and bottom rows of
+ * each ring and counts the centre block twice, as the reference does.
+ * mask as in pargo_idw_run but compared with mask_nodata as a double
+ * (idwTransformation.cpp:13-19); masked cells get `nodata`.  With load_out
+ * NULL only *load_rows / *load_cols are set.  load_out is a device double
+ * array; the call waits for the result and returns PARGO_E_NONTERM when a
+ * cell's walk can never reach k samples (the reference loops forever). */
+int pargo_idw_load(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int granularity,
+                   const int32_t *mask, int mask_rows, int mask_cols, double mask_nodata,
+                   double nodata, double *load_out, int *load_rows, int *load_cols);
+
+/* Splits the rows of a load grid (host array, rows x cols) into nbands
+ * bands of near-equal summed load: DeComposition::valRowDcmp
+ * (gpro/deComposition.h:305-393), which skips cells equal to -9999 or to
+ * `nodata` (one-sided 1e-7 tests) and places each cut greedily.
+ * band_first (host, nbands + 1 entries) receives the first row of every
+ * band and rows as the end.  nbands must be in [1, rows]. */
+int pargo_load_row_bands(const double *load, int rows, int cols, double nodata, int nbands,
+                         int *band_first);
+
 /* ---- adjoints (no counterpart in the reference) ---------------------- */
 
 /* dL/dz of the Horn slope (PARGO_SLOPE_FD, PARGO_UNIT_TAN) given dL/dslope
@@ -206,6 +237,46 @@ int pargo_slope_adjoint(pargo_ctx *ctx, const pargo_grid *grid, const float *dem
 int pargo_idw_adjoint(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
                       const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
                       const float *grad_out, double *grad_vals);
+
+/* ---- files (host only; no GPU needed) --------------------------------- */
+
+/* Raster header of a GeoTIFF: what RasterLayer::readFileInfo takes from
+ * GDAL (gpro/rasterLayer.h:749-778). */
+typedef struct {
+    int32_t rows;
+    int32_t cols;
+    double transform[6]; /* GDAL geotransform */
+    int32_t has_nodata;  /* GDAL_NODATA tag present */
+    double nodata;
+    int32_t bits;          /* bits per sample of the file */
+    int32_t sample_format; /* 1 unsigned, 2 signed, 3 float */
+} pargo_raster_info;
+
+/* Sample types of pargo_geotiff_write. */
+#define PARGO_IO_FLOAT64 0
+#define PARGO_IO_FLOAT32 1
+#define PARGO_IO_INT32 2
+#define PARGO_IO_UINT8 3
+
+/* Header only.  Errors are reported through pargo_last_error(NULL). */
+int pargo_geotiff_info(const char *path, pargo_raster_info *info);
+/* Band 1 widened to double (host array of rows * cols), as GDAL RasterIO
+ * into a double buffer does (rasterLayer.h:882-903).  Reads classic and
+ * BigTIFF, strips or tiles, uncompressed, LZW or PackBits, with or without
+ * a horizontal predictor. */
+int pargo_geotiff_read(const char *path, pargo_raster_info *info, double *data);
+/* Writes one band, one row per strip, uncompressed, with the geotransform,
+ * GDAL_NODATA and, when georef_from names a GeoTIFF, its GeoKeys
+ * (rasterLayer.h:957-1087 writes Float64 for RasterLayer<double>). */
+int pargo_geotiff_write(const char *path, const pargo_raster_info *info, const double *data, int type,
+                        const char *georef_from);
+/* Point shapefile <name>.shp/.dbf: x, y and the DBF field field_index
+ * (0-based, strtod of the field text) of every non-deleted record, the
+ * replacement of IDWOperator::readSamples' OGR calls
+ * (apps/spatial/idw/idwOperator.cpp:63-149).  *n is the room in the output
+ * arrays on entry and the point count on return; with all three arrays
+ * NULL only the count is returned.  A non-point record is an error. */
+int pargo_points_read(const char *shp_path, int field_index, int *n, double *xs, double *ys, double *vals);
 
 /* ---- synthetic inputs (deterministic; SplitMix64 seeded) ------------ */
 

@@ -14,6 +14,9 @@
  *   - mask row/col mapping ....................... gpro/rasterLayer.h:1288-1316
  *   - per-cell virtual Operator + Work loop ...... gpro/rasterOperator.h:147-252
  *   - row bands ................................. gpro/deComposition.h:83-115 (_smpl1DDcmp)
+ *   - IDW load estimate, load-balanced bands ..... apps/spatial/idw/idwTransformation.cpp:3-62,
+ *                                                 gpro/computeLayer.h:208-260,
+ *                                                 gpro/deComposition.h:305-393 (valRowDcmp)
  *   - bounds-checked row access .................. gpro/cellSpace.h:471-491
  *
  * Built with -O2 -ffp-contract=off and without -ffast-math so that every
@@ -596,6 +599,142 @@ int oracle_idw(const double *xs, const double *ys, const double *vals, int n,
     free(count);
     if (o.failed) return -5;
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* IDW compute-load estimate and load-balanced row bands               */
+/* ------------------------------------------------------------------ */
+
+/* IdwTransformation::Operator (apps/spatial/idw/idwTransformation.cpp:3-62)
+ * over every cell of the ComputeLayer built by ComputeLayer::_init
+ * (gpro/computeLayer.h:208-260) from the IDW layer, driven like
+ * Transformation::run (gpro/transformation.h:204-228): row-major, serial.
+ * Returns -3 when a cell's walk can never reach K samples (the reference's
+ * while loop never ends there), -4 when a sample falls outside the block
+ * grid.  With out == NULL only *loadRows / *loadCols are set. */
+int oracle_idw_load(const double *xs, const double *ys, int n, float cellSize, int K, double blockSize,
+                    int granularity, const int32_t *mask, int maskRows, int maskCols, double maskNoData,
+                    double noData, double *out, int *loadRows, int *loadCols)
+{
+    if (n < 1 || K < 1 || blockSize <= 0 || granularity < 1) return -1;
+    double minX, maxX, minY, maxY;
+    int nRows, nCols;
+    oracle_idw_extent(xs, ys, n, cellSize, &minX, &maxX, &minY, &maxY, &nRows, &nCols);
+    /* compute layer: ceil of the IDW work rectangle over the granularity */
+    int lr = nRows / granularity + ((nRows % granularity) ? 1 : 0);
+    int lc = nCols / granularity + ((nCols % granularity) ? 1 : 0);
+    *loadRows = lr;
+    *loadCols = lc;
+    if (!out) return 0;
+    float comptCellSize = cellSize * granularity; /* float * int (computeLayer.h:221) */
+    int blockRows = (int)ceil((maxY - minY) / blockSize);
+    int blockCols = (int)ceil((maxX - minX) / blockSize);
+    int nb = blockRows * blockCols;
+    int *cnt = (int *)calloc((size_t)nb, sizeof(int));
+    for (int i = 0; i < n; i++) {
+        int iCol = (int)((xs[i] - minX) / blockSize), iRow = (int)((maxY - ys[i]) / blockSize);
+        int index = iRow * blockCols + iCol;
+        if (index < 0 || index >= nb) { free(cnt); return -4; }
+        cnt[index]++;
+    }
+    int guard = blockRows > blockCols ? blockRows : blockCols;
+    int rc = 0;
+    for (int cRow = 0; cRow < lr && rc == 0; cRow++) {
+        for (int cCol = 0; cCol < lc; cCol++) {
+            double *o = &out[(size_t)cRow * lc + cCol];
+            if (mask) {
+                /* _pComptLayer->rowAtOtherLayer(maskLayer, cRow) */
+                int maskRow = idw_mask_index(maskRows, lr, cRow);
+                int maskCol = idw_mask_index(maskCols, lc, cCol);
+                int m = mask[(size_t)maskRow * maskCols + maskCol];
+                if ((m == 0) | (m == maskNoData)) { *o = noData; continue; }
+            }
+            /* getBlockRow/ColIndexByCellIndex(i, cellSize) (idwOperator.h:70-71) */
+            int a = (int)((cRow + 0.5) * (double)comptCellSize / blockSize);
+            int b = (int)((cCol + 0.5) * (double)comptCellSize / blockSize);
+            int blockRow = a < blockRows - 1 ? a : blockRows - 1;
+            int blockCol = b < blockCols - 1 ? b : blockCols - 1;
+            int isPointEnough = 0, searchRadius = 0, searchBlockNums = 0;
+            int nbrNums = cnt[blockRow * blockCols + blockCol];
+            while (nbrNums < K) {
+                if (searchRadius > guard) { rc = -3; break; }
+                for (int tRow = blockRow - searchRadius; tRow <= blockRow + searchRadius; ++tRow) {
+                    if (isPointEnough) break;
+                    if (tRow < 0 || tRow >= blockRows) continue;
+                    if (tRow == blockRow - searchRadius || tRow == blockRow + searchRadius) {
+                        for (int tCol = blockCol - searchRadius; tCol <= blockCol + searchRadius; ++tCol) {
+                            if (isPointEnough) break;
+                            if (tCol < 0 || tCol >= blockCols) continue;
+                            nbrNums += cnt[tRow * blockCols + tCol];
+                            searchBlockNums++;
+                            if (nbrNums >= K) isPointEnough = 1;
+                        }
+                    }
+                }
+                if (isPointEnough) break;
+                ++searchRadius;
+            }
+            if (rc) break;
+            double load_1 = searchBlockNums;
+            double load_2 = nbrNums * 2;
+            *o = load_1 + load_2;
+        }
+    }
+    free(cnt);
+    return rc;
+}
+
+/* DeComposition::valRowDcmp (gpro/deComposition.h:305-393).  first[i] is
+ * the first row of band i (nSub + 1 entries, first[nSub] = rows).  The
+ * reference only ever uses the bands' first rows (computeLayer.h:396-428). */
+int oracle_val_row_dcmp(const double *load, int rows, int cols, double noData, int nSubSpcs, int *first)
+{
+    const double Eps = 0.0000001; /* deComposition.h:24 */
+    if (nSubSpcs < 1 || nSubSpcs > rows) return -1;
+    double *rowComptLoad = (double *)malloc(sizeof(double) * (size_t)rows);
+    double totalComptLoad = 0.0;
+    for (int cRow = 0; cRow < rows; cRow++) {
+        rowComptLoad[cRow] = 0.0;
+        for (int cCol = 0; cCol < cols; cCol++) {
+            double v = load[(size_t)cRow * cols + cCol];
+            if ((v + 9999) > Eps && (v - noData) > Eps) rowComptLoad[cRow] += v;
+        }
+        totalComptLoad += rowComptLoad[cRow];
+    }
+    double averComptLoad = totalComptLoad / (nSubSpcs);
+    double accuComptLoad = 0;
+    int subBegin = 0, nb = 0;
+    first[nb++] = 0;
+    for (int i = 0; i < nSubSpcs - 1; ++i) {
+        double subComptLoad = 0.0;
+        double minComptDiff = 0.0;
+        for (int cRow = subBegin; cRow < rows; cRow++) {
+            int rowGap = cRow - 0;
+            subComptLoad += rowComptLoad[rowGap];
+            accuComptLoad += rowComptLoad[rowGap];
+            if (subComptLoad <= averComptLoad && rows - 1 - cRow > nSubSpcs - 1 - i) {
+                if (fabs(minComptDiff) < Eps || (averComptLoad - subComptLoad) < minComptDiff) {
+                    minComptDiff = averComptLoad - subComptLoad;
+                }
+            } else {
+                if (fabs(minComptDiff) < Eps || (subComptLoad - averComptLoad) <= minComptDiff ||
+                    rows - 1 - cRow == nSubSpcs - 1 - i) {
+                    averComptLoad = (totalComptLoad - accuComptLoad) / (nSubSpcs - i - 1);
+                    subBegin = cRow + 1;
+                } else {
+                    accuComptLoad -= rowComptLoad[rowGap];
+                    subComptLoad -= rowComptLoad[rowGap];
+                    averComptLoad = (totalComptLoad - accuComptLoad) / (nSubSpcs - i - 1);
+                    subBegin = cRow;
+                }
+                first[nb++] = subBegin;
+                break;
+            }
+        }
+    }
+    free(rowComptLoad);
+    first[nSubSpcs] = rows;
+    return nb == nSubSpcs ? 0 : -2;
 }
 
 /* ------------------------------------------------------------------ */

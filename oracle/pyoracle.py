@@ -31,6 +31,10 @@ def lib():
                                       ctypes.POINTER(ctypes.c_double)]
         L.oracle_idw.argtypes = [P, P, P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_double, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
+        L.oracle_idw_load.argtypes = [P, P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.oracle_val_row_dcmp.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
         L.oracle_div_sqrt2.argtypes = [P, P, ctypes.c_int64]
         L.oracle_smpl1d_dcmp.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2
         L.oracle_num_threads.restype = ctypes.c_int
@@ -102,6 +106,40 @@ def idw(xs, ys, vals, cellsize, k=12, power=2, buffer=0.0, block_size=5000.0, ma
     if stats:
         return out, cand, blk
     return out
+
+
+def idw_load(xs, ys, cellsize, k=12, block_size=5000.0, granularity=10, mask=None, mask_nodata=0.0,
+             nodata=-9999.0):
+    """IDW compute-load grid (IdwTransformation over a ComputeLayer)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    r, c = ctypes.c_int(), ctypes.c_int()
+    m = None
+    mr = mc = 0
+    if mask is not None:
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        mr, mc = m.shape
+    L = lib()
+    args = [_p(xs), _p(ys), len(xs), float(cellsize), int(k), float(block_size), int(granularity), _p(m), mr, mc,
+            float(mask_nodata), float(nodata)]
+    rc = L.oracle_idw_load(*args, None, ctypes.byref(r), ctypes.byref(c))
+    if rc != 0:
+        raise ValueError("oracle_idw_load failed: %d" % rc)
+    out = np.empty((r.value, c.value), dtype=np.float64)
+    rc = L.oracle_idw_load(*args, _p(out), ctypes.byref(r), ctypes.byref(c))
+    if rc != 0:
+        raise ValueError("oracle_idw_load failed: %d" % rc)
+    return out
+
+
+def val_row_dcmp(load, nodata, nsub):
+    """First row of each of nsub load-balanced bands, plus rows at the end."""
+    load = np.ascontiguousarray(load, dtype=np.float64)
+    first = np.empty(nsub + 1, dtype=np.int32)
+    rc = lib().oracle_val_row_dcmp(_p(load), load.shape[0], load.shape[1], float(nodata), int(nsub), _p(first))
+    if rc != 0:
+        raise ValueError("oracle_val_row_dcmp failed: %d" % rc)
+    return first
 
 
 def div_sqrt2(x):

@@ -43,11 +43,20 @@ _SIGS = {
     "pargo_slope_d8": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _P, _P, _P]),
     "pargo_idw_grid": (_c.c_int, [_P, _P, _c.c_int, _c.c_float, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int),
                                   _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "pargo_idw_extent": (_c.c_int, [_P, _P, _c.c_int, _c.c_float, _P, _c.POINTER(_c.c_int),
+                                    _c.POINTER(_c.c_int)]),
     "pargo_idw_prepare": (_c.c_int, [_P, _P, _P, _P, _c.c_int, _c.c_float, _c.c_double, _c.POINTER(_P)]),
     "pargo_idw_plan_free": (_c.c_int, [_P, _P]),
     "pargo_idw_set_values": (_c.c_int, [_P, _P, _P]),
     "pargo_idw_run": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int, _c.c_int,
                                  _P, _P]),
+    "pargo_idw_load": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
+                                  _P, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
+    "pargo_load_row_bands": (_c.c_int, [_P, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _P]),
+    "pargo_geotiff_info": (_c.c_int, [_c.c_char_p, _P]),
+    "pargo_geotiff_read": (_c.c_int, [_c.c_char_p, _P, _P]),
+    "pargo_geotiff_write": (_c.c_int, [_c.c_char_p, _P, _P, _c.c_int, _c.c_char_p]),
+    "pargo_points_read": (_c.c_int, [_c.c_char_p, _c.c_int, _c.POINTER(_c.c_int), _P, _P, _P]),
     "pargo_slope_adjoint": (_c.c_int, [_P, _P, _P, _P, _P]),
     "pargo_idw_adjoint": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
                                      _c.c_int, _P, _P]),
@@ -67,6 +76,21 @@ class PargoGrid(ctypes.Structure):
     """pargo_grid (include/pargo.h)."""
     _fields_ = [("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("pitch", ctypes.c_int64),
                 ("cellsize", ctypes.c_double), ("nodata", ctypes.c_double)]
+
+
+class RasterInfo(ctypes.Structure):
+    """pargo_raster_info (include/pargo.h)."""
+    _fields_ = [("rows", ctypes.c_int32), ("cols", ctypes.c_int32), ("transform", ctypes.c_double * 6),
+                ("has_nodata", ctypes.c_int32), ("nodata", ctypes.c_double), ("bits", ctypes.c_int32),
+                ("sample_format", ctypes.c_int32)]
+
+
+def check_host(rc, what):
+    """Raises PargoError for a failed context-free (host-only) call."""
+    if rc != PARGO_OK:
+        msg = lib().pargo_last_error(None)
+        raise PargoError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+    return rc
 
 
 class PargoError(RuntimeError):

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-ffp-contract=off"]
 
 HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip"]
-CXX_SOURCES = ["synth.cpp"]
+CXX_SOURCES = ["synth.cpp", "dcmp.cpp", "io.cpp", "../gpro/geotiff.cpp", "../gpro/shapefile.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -48,7 +48,7 @@ def _run(cmd, cwd=None):
 
 def _compile(src):
     path = os.path.join(CSRC, src)
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    out = os.path.join(OBJ, os.path.splitext(os.path.basename(src))[0] + ".o")
     if src.endswith(".hip"):
         cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", out]
     else:
@@ -68,6 +68,50 @@ def build_library(jobs=8):
     return os.path.join(BUILD, "libpargo.so")
 
 
+GPRO = os.path.join(PKG, "gpro")
+# the GeoTIFF and shapefile codecs are part of libpargo.so (pargo_geotiff_*)
+GPRO_SOURCES = ["application.cpp", "focalDevice.cpp"]
+# command line -> (app directory, sources)
+APP_SOURCES = {
+    "slope": ("slope", ["slopeOperator.cpp", "pSlope.cpp"]),
+    "flowdird8": ("flowdird8", ["d8Operator.cpp", "pD8.cpp"]),
+    "idw": ("idw", ["idwOperator.cpp", "idwTransformation.cpp", "pIDW.cpp"]),
+    # host-only checks of the gpro layer, run by tests/test_gpro_host.py
+    "gpro_host_test": ("../../tests/cpp", ["gpro_host_test.cpp"]),
+}
+APP_FLAGS = ["-O2", "-std=c++17", "-fopenmp", "-Wall", "-ffp-contract=off"]
+
+
+def _compile_host(src, out):
+    _run([CXX] + APP_FLAGS + ["-c", src, "-o", out])
+    return out
+
+
+def build_apps(jobs=8):
+    """The gpro host layer and the slope / flowdird8 / idw command lines,
+    linked against libpargo.so next to them (rpath $ORIGIN/..)."""
+    aobj = os.path.join(OBJ, "apps")
+    bindir = os.path.join(BUILD, "bin")
+    os.makedirs(aobj, exist_ok=True)
+    os.makedirs(bindir, exist_ok=True)
+    jobs_list = [(os.path.join(GPRO, s), os.path.join(aobj, "gpro_" + s[:-4] + ".o")) for s in GPRO_SOURCES]
+    for name, (d, srcs) in APP_SOURCES.items():
+        jobs_list += [(os.path.normpath(os.path.join(APPS, d, s)),
+                       os.path.join(aobj, name + "_" + os.path.basename(s)[:-4] + ".o")) for s in srcs]
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(lambda a: _compile_host(*a), jobs_list))
+    gobjs = [os.path.join(aobj, "gpro_" + s[:-4] + ".o") for s in GPRO_SOURCES]
+    outs = []
+    for name, (d, srcs) in APP_SOURCES.items():
+        objs = [os.path.join(aobj, name + "_" + os.path.basename(s)[:-4] + ".o") for s in srcs]
+        exe = os.path.join(bindir, name)
+        _run([CXX, "-o", exe + ".tmp"] + objs + gobjs +
+             ["-L" + BUILD, "-lpargo", "-fopenmp", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(exe + ".tmp", exe)
+        outs.append(exe)
+    return outs
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "_build", "liboracle.so")
@@ -79,6 +123,8 @@ def main(argv=None):
     ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args(argv)
     print(build_library(a.jobs))
+    for exe in build_apps(a.jobs):
+        print(exe)
     if not a.no_oracle:
         print(build_oracle())
 

@@ -290,6 +290,64 @@ __global__ void idw_unbucket_kernel(const double *__restrict__ acc, const int *_
     if (q < n) out[perm[q]] = acc[q];
 }
 
+// Compute-load estimate of one coarse cell (IdwTransformation::Operator,
+// idwTransformation.cpp:3-62): the sample count of the cell's block, then
+// rings r = 0, 1, ... of which only the full top and bottom block rows are
+// visited (the centre block again at r = 0), until K samples are counted.
+// load = blocks visited + 2 * samples counted.
+struct IdwLoadParams {
+    const int *__restrict__ start;
+    const int32_t *__restrict__ mask;
+    double *__restrict__ out;
+    unsigned *__restrict__ nonterm;
+    double block, ccs, mask_nodata, nodata;
+    int rows, cols, blockRows, blockCols, k, maxRing;
+    int mask_rows, mask_cols;
+};
+
+__global__ __launch_bounds__(256) void idw_load_kernel(IdwLoadParams p)
+{
+    const int cCol = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int cRow = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (cRow >= p.rows || cCol >= p.cols) return;
+    double *o = p.out + (long long)cRow * p.cols + cCol;
+    if (p.mask) {
+        const int mr = mask_index(p.mask_rows, p.rows, cRow);
+        const int mc = mask_index(p.mask_cols, p.cols, cCol);
+        const int m = p.mask[(long long)mr * p.mask_cols + mc];
+        if (m == 0 || (double)m == p.mask_nodata) {
+            *o = p.nodata;
+            return;
+        }
+    }
+    // getBlockRowIndexByCellIndex(iRow, cellSize) (idwOperator.h:70-71)
+    const int bR = min((int)((cRow + 0.5) * p.ccs / p.block), p.blockRows - 1);
+    const int bC = min((int)((cCol + 0.5) * p.ccs / p.block), p.blockCols - 1);
+    auto count = [&](int r, int c) { return p.start[r * p.blockCols + c + 1] - p.start[r * p.blockCols + c]; };
+    long long nbr = count(bR, bC);
+    long long blocks = 0;
+    bool enough = false;
+    for (int rad = 0; nbr < p.k; rad++) {
+        if (rad > p.maxRing) {  // no block is left to add: the reference never returns
+            atomicAdd(p.nonterm, 1u);
+            *o = p.nodata;
+            return;
+        }
+        for (int tRow = bR - rad; tRow <= bR + rad && !enough; ++tRow) {
+            if (tRow < 0 || tRow >= p.blockRows) continue;
+            if (tRow != bR - rad && tRow != bR + rad) continue;
+            for (int tCol = bC - rad; tCol <= bC + rad && !enough; ++tCol) {
+                if (tCol < 0 || tCol >= p.blockCols) continue;
+                nbr += count(tRow, tCol);
+                blocks++;
+                if (nbr >= p.k) enough = true;
+            }
+        }
+        if (enough) break;
+    }
+    *o = (double)blocks + (double)(nbr * 2);
+}
+
 int idw_fill_params(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
                     const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, IdwParams &p)
 {
@@ -339,6 +397,14 @@ int pargo_idw_grid(const double *xs, const double *ys, int n, float cellsize, in
     pargo::idw_extent(xs, ys, n, cellsize, a, b, c, d, *rows, *cols);
     if (min_x) *min_x = a;
     if (max_y) *max_y = d;
+    return PARGO_OK;
+}
+
+int pargo_idw_extent(const double *xs, const double *ys, int n, float cellsize, double extent[4], int *rows,
+                     int *cols)
+{
+    if (!xs || !ys || n < 1 || !(cellsize > 0) || !extent || !rows || !cols) return PARGO_E_INVALID;
+    pargo::idw_extent(xs, ys, n, cellsize, extent[0], extent[1], extent[2], extent[3], *rows, *cols);
     return PARGO_OK;
 }
 
@@ -467,6 +533,57 @@ int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, 
     rc = pargo::launch_idw<false>(p, ctx->stream);
     if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
     return pargo::check_launch(ctx, "pargo_idw_run");
+}
+
+int pargo_idw_load(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int granularity, const int32_t *mask,
+                   int mask_rows, int mask_cols, double mask_nodata, double nodata, double *load_out,
+                   int *load_rows, int *load_cols)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!plan || plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "IDW plan is null or from another context");
+    if (k < 1) return fail(ctx, PARGO_E_INVALID, "searchPointNum must be >= 1, got %d", k);
+    if (granularity < 1) return fail(ctx, PARGO_E_INVALID, "granularity must be >= 1, got %d", granularity);
+    if (mask && (mask_rows < 1 || mask_cols < 1)) return fail(ctx, PARGO_E_INVALID, "bad mask dimensions");
+    // ComputeLayer::_init (gpro/computeLayer.h:208-260): ceil of the work
+    // rectangle over the granularity, cell size cellSize * granularity in float
+    const int rows = plan->rows / granularity + (plan->rows % granularity ? 1 : 0);
+    const int cols = plan->cols / granularity + (plan->cols % granularity ? 1 : 0);
+    if (load_rows) *load_rows = rows;
+    if (load_cols) *load_cols = cols;
+    if (!load_out) return PARGO_OK;
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    rc = pargo::ensure_scratch(ctx, sizeof(unsigned));
+    if (rc) return rc;
+    pargo::IdwLoadParams p{};
+    p.start = plan->d_start;
+    p.mask = mask;
+    p.out = load_out;
+    p.nonterm = (unsigned *)ctx->scratch;
+    p.block = plan->block;
+    p.ccs = (double)(plan->cs * (float)granularity);
+    p.mask_nodata = mask_nodata;
+    p.nodata = nodata;
+    p.rows = rows;
+    p.cols = cols;
+    p.blockRows = plan->blockRows;
+    p.blockCols = plan->blockCols;
+    p.k = k;
+    p.maxRing = std::max(plan->blockRows, plan->blockCols);
+    p.mask_rows = mask_rows;
+    p.mask_cols = mask_cols;
+    PARGO_HIP(ctx, hipMemsetAsync(p.nonterm, 0, sizeof(unsigned), ctx->stream));
+    hipLaunchKernelGGL(pargo::idw_load_kernel, dim3((cols + 15) / 16, (rows + 15) / 16), dim3(256), 0, ctx->stream, p);
+    rc = pargo::check_launch(ctx, "pargo_idw_load");
+    if (rc) return rc;
+    unsigned bad = 0;
+    PARGO_HIP(ctx, hipMemcpyAsync(&bad, p.nonterm, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (bad)
+        return fail(ctx, PARGO_E_NONTERM,
+                    "%u compute cells can never count searchPointNum %d samples: the reference's estimate never ends",
+                    bad, k);
+    return PARGO_OK;
 }
 
 int pargo_idw_adjoint(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,

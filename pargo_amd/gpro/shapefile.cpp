@@ -74,7 +74,7 @@ bool read_points(const std::string &path, int field_index, Points &out, std::str
         fields.push_back({off, dbf[o + 16], (char)dbf[o + 11]});
         off += dbf[o + 16];
     }
-    if (field_index < 0 || field_index >= (int)fields.size()) {
+    if (field_index >= (int)fields.size()) {
         err = "field index " + std::to_string(field_index) + " out of range (" + std::to_string(fields.size()) +
               " fields)";
         return false;
@@ -106,6 +106,10 @@ bool read_points(const std::string &path, int field_index, Points &out, std::str
         }
         out.x.push_back(led(c + 4));
         out.y.push_back(led(c + 12));
+        if (field_index < 0) {
+            out.value.push_back(0.0);
+            continue;
+        }
         const Field &f = fields[field_index];
         std::string txt((const char *)r + f.off, (size_t)f.len);
         out.value.push_back(std::strtod(txt.c_str(), nullptr));

@@ -20,7 +20,8 @@ struct Points {
     std::vector<std::string> field_names;
 };
 
-// `path` is the .shp file.  On a non-point record the reference prints
+// `path` is the .shp file; a negative field_index reads no values (zeros).
+// On a non-point record the reference prints
 // "[ERROR] No point geometry" and stops reading; this returns false.
 bool read_points(const std::string &path, int field_index, Points &out, std::string &err);
 

@@ -26,6 +26,17 @@ def grid(xs, ys, cellsize):
     return r.value, c.value, mx.value, my.value
 
 
+def row_bands(load, nbands, nodata=-9999.0):
+    """First row of each of nbands load-balanced row bands of a host load
+    grid, plus the row count (DeComposition::valRowDcmp,
+    gpro/deComposition.h:305-393)."""
+    load = np.ascontiguousarray(load, dtype=np.float64)
+    first = np.empty(int(nbands) + 1, dtype=np.int32)
+    _lib.check_host(_lib.lib().pargo_load_row_bands(_lib._ptr(load), load.shape[0], load.shape[1], float(nodata),
+                                                    int(nbands), _lib._ptr(first)), "pargo_load_row_bands")
+    return first
+
+
 class IDWOperator:
     def __init__(self, cellsize, nbr_points, idw_power, buffer_size, block_size, ctx=None, device=0):
         self.cellsize = float(np.float32(cellsize))
@@ -78,6 +89,26 @@ class IDWOperator:
                                                 _lib._ptr(cand)))
         if count_candidates:
             return out, cand
+        return out
+
+    def compute_load(self, granularity=10, mask=None, mask_nodata=0.0, nodata=-9999.0):
+        """Estimated load per granularity x granularity block of cells
+        (IdwTransformation over a ComputeLayer, idwTransformation.cpp:3-62),
+        a float64 device tensor.  mask is a device int32 grid compared with
+        mask_nodata as a double, as the reference's estimate does."""
+        import torch
+        L = _lib.lib()
+        r, c = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(L.pargo_idw_load(self.ctx.h, self.plan, self.k, int(granularity), None, 0, 0, 0.0,
+                                        float(nodata), None, ctypes.byref(r), ctypes.byref(c)))
+        out = torch.empty((r.value, c.value), dtype=torch.float64, device=torch.device("cuda", self.ctx.device))
+        mr = mc = 0
+        if mask is not None:
+            mr, mc = mask.shape
+        self.ctx.bind_stream()
+        self.ctx.check(L.pargo_idw_load(self.ctx.h, self.plan, self.k, int(granularity), _lib._ptr(mask), mr, mc,
+                                        float(mask_nodata), float(nodata), _lib._ptr(out), ctypes.byref(r),
+                                        ctypes.byref(c)))
         return out
 
     def adjoint(self, grad_out):

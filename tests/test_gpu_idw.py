@@ -122,3 +122,26 @@ def test_idw_medium_full_recipe():
     # 1024 x 1024 grid of the benchmark recipe (30 m cells, 5 km blocks/buffer)
     xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 5000, width=30720.0, height=30720.0)
     compare(xs, ys, vs, 30.0, 12, 2, 5000.0, 5000.0)
+
+
+@pytest.mark.parametrize("k,g,block", [(12, 10, 700.0), (4, 3, 400.0), (30, 16, 1500.0)])
+def test_compute_load_matches_oracle(k, g, block):
+    import torch
+    xs, ys, vs = rand_samples(500, 77 + k, 6000.0)
+    op = IDWOperator(37.0, k, 2, 0.0, block).read_samples(xs, ys, vs)
+    rows, cols = op.rows, op.cols
+    mask = (np.random.default_rng(k).random((rows // 2 + 3, cols // 3 + 2)) < 0.8).astype(np.int32) * 5
+    mask[0, 0] = 9
+    got = op.compute_load(g, torch.from_numpy(mask).cuda(), mask_nodata=9.0).cpu().numpy()
+    op.free()
+    ref = pyoracle.idw_load(xs, ys, 37.0, k=k, block_size=block, granularity=g, mask=mask, mask_nodata=9.0)
+    assert np.array_equal(got, ref)
+
+
+def test_compute_load_nonterminating_is_rejected():
+    xs, ys, vs = rand_samples(5, 3, 6000.0)
+    op = IDWOperator(37.0, 12, 2, 0.0, 1000.0).read_samples(xs, ys, vs)
+    with pytest.raises(_lib.PargoError) as e:
+        op.compute_load(10)
+    op.free()
+    assert e.value.code == _lib.PARGO_E_NONTERM
