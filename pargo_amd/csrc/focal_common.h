@@ -20,7 +20,8 @@ constexpr double kD8Eps = 0.000001;
 // |y*s - 1| = 0.183 * 2^-53; by Markstein's theorem (y within half an ulp
 // of 1/s, q0 within one ulp of x/s) r = x - q0*s is exact in one FMA and
 // RN(q0 + r*y) is the correctly rounded quotient.  The identity is also
-// checked on the GPU against IEEE division by the test suite.
+// checked on the host against IEEE division over every binade from 2^-149
+// to 2^129 (tests/cpp/gpro_host_test.cpp).
 __device__ __forceinline__ double div_sqrt2(double x)
 {
     double q0 = x * kInvSqrt2;

@@ -3,7 +3,10 @@
 // reads, rowAtOtherLayer, ComputeLayer dimensions and load bands.
 // Usage: gpro_host_test <dir with dem.tif mask_500.tif *.nbr> <scratch dir>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <random>
 #include <sstream>
 
 #include "../../pargo_amd/gpro/computeLayer.h"
@@ -138,6 +141,33 @@ int main(int argc, char **argv)
         CHECK(bands.size() == 2 && bands[0].minIRow() == 0 && bands[1].maxIRow() == 698);
         CHECK(bands[1].minIRow() == (int)(36 * (699 / 70.0)) && bands[0].maxIRow() == bands[1].minIRow() - 1);
         CHECK(!cl.getCompuLoadBands(71, bands));
+    }
+    // The D8 kernels divide by sqrt(2.0) with q0 = x*y; r = fma(-q0, S, x);
+    // q = fma(r, y, q0) (pargo_amd/csrc/focal_common.h).  Check that this
+    // equals IEEE x / sqrt(2.0) over every binade that a difference of two
+    // float32 values can occupy (2^-149 .. 2^129), with random mantissas.
+    {
+        const double S = 0x1.6a09e667f3bcdp+0, Y = 0x1.6a09e667f3bccp-1;
+        CHECK(S == std::sqrt(2.0));
+        std::mt19937_64 rng(12345);
+        long long bad = 0, n = 0;
+        for (int e = -149; e <= 129; e++) {
+            for (int i = 0; i < 4096; i++) {
+                const uint64_t mant = rng() & ((1ull << 52) - 1);
+                double x = std::ldexp(1.0 + std::ldexp((double)mant, -52), e);
+                if (i & 1) x = -x;
+                if (i == 0) x = std::ldexp(1.0, e);
+                const volatile double xv = x;
+                const double q0 = xv * Y;
+                const double r = std::fma(-q0, S, xv);
+                const double q = std::fma(r, Y, q0);
+                const volatile double ref = xv / std::sqrt(2.0);
+                bad += (q != ref);
+                n++;
+            }
+        }
+        CHECK(bad == 0);
+        std::printf("division by sqrt(2): %lld of %lld values differ from IEEE\n", bad, n);
     }
     if (failures) std::fprintf(stderr, "%d check(s) failed\n", failures);
     else std::printf("gpro host checks passed\n");
