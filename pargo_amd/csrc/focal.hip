@@ -84,10 +84,11 @@ struct FocalParams {
     int unit;
     int dirtype;          // 0: no D8, 8 or 256
     int mask_from_d8;     // NoData mask reports the D8 NoData set
-    // Cells whose window holds Inf or |z| >= 2^40 are queued here and
-    // recomputed by focal_fixup_kernel with the reference-order FP64 code.
-    unsigned *fix_count;  // [0] queued cells, [1] finished fixup blocks
-    uint2 *fix_list;      // (tile index, linear cell index r * cols + c)
+    // Strips (one wave's columns over a row strip) with a window that holds
+    // Inf or |z| >= 2^40 are queued here; focal_fixup_kernel recomputes those
+    // windows with the reference-order FP64 code.
+    unsigned *fix_count;  // [0] queued strips, [1] finished fixup blocks
+    uint4 *fix_list;      // (tile index, first row, end row, first column)
     unsigned fix_cap;
     unsigned tile_id;     // index of this raster in a batch (0 otherwise)
     int vec;              // 16-byte aligned rows: vector loads and stores
@@ -100,33 +101,30 @@ struct RawRow {
     float hl;  // outer halo column: c0-1 on lane 0, c0+CPT on lane 63
 };
 
-// `full`: the whole wave's columns are inside the raster (wave-uniform).
-// Every lane issues the halo load; lane 0 fetches column c0-1, lane 63
-// column c0+CPT and the others re-read a column of their own segment (an L1
-// hit), which is cheaper than branching on the lane id.
-template <int CPT, bool VEC>
-__device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, int lane, bool full, RawRow<CPT> &o)
+// Loads DEM row r (clamped into the raster) for the lane's CPT columns and
+// its outer halo column.  Every load is unconditional: rows above the top or
+// below the bottom and columns past the right edge read the nearest row or
+// column instead, which only ever feeds border cells (NoData anyway) or
+// columns that are not stored.  Keeping the loads out of branches lets the
+// compiler wait only for the oldest of the rows in flight.
+// VECLOAD: the wave's 64*CPT columns are inside the raster and the row start
+// is 16-byte aligned (one dwordx4 / dwordx2 per lane).  The halo load is
+// issued by every lane: lane 0 fetches column c0-1, lane 63 column c0+CPT,
+// the others re-read a column of their own segment (an L1 hit).
+template <int CPT, bool VECLOAD>
+__device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, int lane, RawRow<CPT> &o)
 {
-#pragma unroll
-    for (int k = 0; k < CPT; k++) o.v[k] = 0.f;
-    o.hl = 0.f;
-    if (r < 0 || r >= p.rows) return;  // wave-uniform
-    const float *rowp = p.dem + (long long)r * p.pitch;
-    if (VEC && full) {
-        if constexpr (CPT == 4) {
-            float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
-            o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
-        } else if constexpr (CPT == 2) {
-            float2 t = *reinterpret_cast<const float2 *>(rowp + c0);
-            o.v[0] = t.x; o.v[1] = t.y;
-        } else {
-#pragma unroll
-            for (int k = 0; k < CPT; k++) o.v[k] = rowp[c0 + k];
-        }
+    const int rr = r < 0 ? 0 : (r >= p.rows ? p.rows - 1 : r);
+    const float *rowp = p.dem + (long long)rr * p.pitch;
+    if constexpr (VECLOAD && CPT == 4) {
+        float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
+        o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
+    } else if constexpr (VECLOAD && CPT == 2) {
+        float2 t = *reinterpret_cast<const float2 *>(rowp + c0);
+        o.v[0] = t.x; o.v[1] = t.y;
     } else {
 #pragma unroll
-        for (int k = 0; k < CPT; k++)
-            if (c0 + k < p.cols) o.v[k] = rowp[c0 + k];
+        for (int k = 0; k < CPT; k++) o.v[k] = rowp[min(c0 + k, p.cols - 1)];
     }
     int hc = (lane == 0) ? c0 - 1 : (lane == 63 ? c0 + CPT : c0);
     hc = hc < 0 ? 0 : (hc > p.cols - 1 ? p.cols - 1 : hc);
@@ -186,8 +184,10 @@ struct Slot {
     double V[CPT];       // RN(z[j+1] - below[j+1])
     double SE[CPT + 1];  // G(z[i] - below[i+1]), i = 0..CPT
     double SW[CPT + 2];  // G(z[i] - below[i-1]), i = 1..CPT+1
-    bool exc;            // NoData or a special value among this lane's values
-    unsigned nd, nd8, sp;  // per-position bits, filled only when exc
+    // per-position bits of this lane's CPT+2 values, filled only for rows
+    // holding NoData or a special value: slope NoData (bits 0-7), D8 NoData
+    // (8-15), |z| >= 2^40 or Inf (16-23)
+    unsigned flags;
 };
 
 // Converts a loaded row.  The common case costs a min, a max and a max of
@@ -212,19 +212,17 @@ __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CP
         mx = fmaxf(mx, f[i]);
         am = fmaxf(am, __builtin_fabsf(f[i]));
     }
-    o.exc = !(mx < p.ndu_lo || mn > p.ndu_hi) || !(am < 0x1p40f);
-    o.nd = o.nd8 = o.sp = 0;
-    if (o.exc) {
-        unsigned nd = 0, nd8 = 0, sp = 0;
+    const bool exc = !(mx < p.ndu_lo || mn > p.ndu_hi) || !(am < 0x1p40f);
+    o.flags = 0;
+    if (exc) {
+        unsigned fl = 0;
 #pragma unroll
         for (int i = 0; i < CPT + 2; i++) {
-            nd |= (unsigned)in_interval(f[i], p.nd_lo, p.nd_hi) << i;
-            nd8 |= (unsigned)in_interval(f[i], p.nd8_lo, p.nd8_hi) << i;
-            sp |= (unsigned)(!(__builtin_fabsf(f[i]) < 0x1p40f)) << i;
+            fl |= (unsigned)in_interval(f[i], p.nd_lo, p.nd_hi) << i;
+            fl |= (unsigned)in_interval(f[i], p.nd8_lo, p.nd8_hi) << (i + 8);
+            fl |= (unsigned)(!(__builtin_fabsf(f[i]) < 0x1p40f)) << (i + 16);
         }
-        o.nd = nd;
-        o.nd8 = nd8;
-        o.sp = sp;
+        o.flags = fl;
     }
 #pragma unroll
     for (int i = 0; i < CPT + 2; i++) o.z[i] = (double)f[i];
@@ -301,13 +299,15 @@ __device__ __forceinline__ float slope_unit(float s, int unit)
 }
 
 // Output row r with up = row r-1, mid = row r, dn = the slot that receives
-// row r+1 from `raw`; `raw` is then refilled with row r+4.
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
-__device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, int lane, bool full, unsigned colb,
-                                           Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw)
+// row r+1 from `raw`; `raw` is then refilled with row r+4.  `special` is set
+// when a window of this lane holds a value the fast path does not handle.
+template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN>
+__device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, int lane, unsigned colb,
+                                           Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw,
+                                           bool &special)
 {
     finish_row<CPT>(p, raw, lane, dn);
-    load_raw<CPT, VEC>(p, r + 4, c0, lane, full, raw);
+    load_raw<CPT, VECLOAD>(p, r + 4, c0, lane, raw);
 
     float sl[CPT];
     unsigned cd[CPT];
@@ -374,26 +374,20 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
     }
 
     // ---- NoData, border and special windows (rare) -------------------------
-    // (the queued special cells keep their fast-path values until the fixup)
+    // (windows with a special value keep their fast-path values until the
+    // fixup pass recomputes them)
     const bool rowb = (r == 0) || (r == p.rows - 1);
-    const bool wexc = up.exc | mid.exc | dn.exc | (colb != 0) | rowb;
+    const unsigned flw = up.flags | mid.flags | dn.flags;
     unsigned ndflags = 0;  // bit j: slope NoData; bit j+8: D8 NoData
-    if (__builtin_expect(wexc, 0)) {
-        const unsigned spw = up.sp | mid.sp | dn.sp;
-        const unsigned ndw = up.nd | mid.nd | dn.nd;
-        const unsigned nd8w = up.nd8 | mid.nd8 | dn.nd8;
+    if (__builtin_expect((flw != 0) | (colb != 0) | rowb, 0)) {
 #pragma unroll
         for (int j = 0; j < CPT; j++) {
             const bool border = rowb || ((colb >> j) & 1u);
-            const bool nds = border || ((ndw >> j) & 7u), n8 = border || ((nd8w >> j) & 7u);
+            const bool nds = border || ((flw >> j) & 7u), n8 = border || ((flw >> (j + 8)) & 7u);
             ndflags |= (unsigned)nds << j;
             ndflags |= (unsigned)n8 << (j + 8);
-            // a special value in a window that is not NoData for every output:
-            // queue the cell for the exact pass
-            if (((spw >> j) & 7u) && !((!DO_SLOPE || nds) && (!DO_D8 || n8))) {
-                const unsigned k = atomicAdd(p.fix_count, 1u);
-                if (k < p.fix_cap) p.fix_list[k] = make_uint2(p.tile_id, (unsigned)r * (unsigned)p.cols + (unsigned)(c0 + j));
-            }
+            // a special value in a window that is not NoData for every output
+            special |= ((flw >> (j + 16)) & 7u) && !((!DO_SLOPE || nds) && (!DO_D8 || n8));
         }
     }
     if (DO_SLOPE && p.unit == PARGO_UNIT_DEGREES) {
@@ -419,7 +413,7 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
     // ---- stores --------------------------------------------------------------
     const long long base = (long long)r * p.pitch + c0;
     if (DO_SLOPE) {
-        if (VEC && full) {
+        if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
                 *reinterpret_cast<float4 *>(p.slope + base) = make_float4(sl[0], sl[1], sl[2], sl[3]);
             } else if constexpr (CPT == 2) {
@@ -435,7 +429,7 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
         }
     }
     if (DO_D8) {
-        if (VEC && full) {
+        if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
                 *reinterpret_cast<unsigned *>(p.code + base) = cd[0] | (cd[1] << 8) | (cd[2] << 16) | (cd[3] << 24);
             } else if constexpr (CPT == 2) {
@@ -476,11 +470,10 @@ __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
 
 // One wave computes output rows [rbeg, rend) of the 64*CPT columns that
 // start at wave_c0 (wave-uniform).
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
-__device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
+template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN>
+__device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
 {
     const int c0 = wave_c0 + lane * CPT;
-    const bool full = wave_c0 + 64 * CPT <= p.cols;
     unsigned colb = 0;  // bit j: column c0+j is on (or past) the border
 #pragma unroll
     for (int j = 0; j < CPT; j++) colb |= (unsigned)(c0 + j == 0 || c0 + j >= p.cols - 1) << j;
@@ -489,23 +482,43 @@ __device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, i
     RawRow<CPT> R0, R1, R2;
     {
         RawRow<CPT> ra, rb;
-        load_raw<CPT, VEC>(p, rbeg - 1, c0, lane, full, ra);
-        load_raw<CPT, VEC>(p, rbeg, c0, lane, full, rb);
-        load_raw<CPT, VEC>(p, rbeg + 1, c0, lane, full, R0);
-        load_raw<CPT, VEC>(p, rbeg + 2, c0, lane, full, R1);
-        load_raw<CPT, VEC>(p, rbeg + 3, c0, lane, full, R2);
+        load_raw<CPT, VECLOAD>(p, rbeg - 1, c0, lane, ra);
+        load_raw<CPT, VECLOAD>(p, rbeg, c0, lane, rb);
+        load_raw<CPT, VECLOAD>(p, rbeg + 1, c0, lane, R0);
+        load_raw<CPT, VECLOAD>(p, rbeg + 2, c0, lane, R1);
+        load_raw<CPT, VECLOAD>(p, rbeg + 3, c0, lane, R2);
         finish_row<CPT>(p, ra, lane, A);
         finish_row<CPT>(p, rb, lane, B);
     }
     pair_rows<CPT, DO_SLOPE, DO_D8, HORN>(A, B);
+    bool special = false;
     // Three slots rotate roles statically: (up, mid, dn) = (A,B,C), (B,C,A), (C,A,B).
     for (int r = rbeg; r < rend; r += 3) {
-        focal_step<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, r, c0, lane, full, colb, A, B, C, R0);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN>(p, r, c0, lane, colb, A, B, C, R0, special);
         if (r + 1 >= rend) break;
-        focal_step<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, r + 1, c0, lane, full, colb, B, C, A, R1);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN>(p, r + 1, c0, lane, colb, B, C, A, R1, special);
         if (r + 2 >= rend) break;
-        focal_step<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, r + 2, c0, lane, full, colb, C, A, B, R2);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN>(p, r + 2, c0, lane, colb, C, A, B, R2, special);
     }
+    // queue the strip for the exact pass (rare: Inf or |z| >= 2^40 nearby)
+    if (__builtin_amdgcn_ballot_w64(special) != 0 && lane == 0) {
+        const unsigned k = atomicAdd(p.fix_count, 1u);
+        if (k < p.fix_cap) p.fix_list[k] = make_uint4(p.tile_id, (unsigned)rbeg, (unsigned)rend, (unsigned)wave_c0);
+    }
+}
+
+// VEC: the raster's pitch and base allow vector access; a wave uses it when
+// all its columns are inside the raster.
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+__device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
+{
+    if constexpr (VEC) {
+        if (wave_c0 + 64 * CPT <= p.cols) {
+            focal_strip_impl<CPT, true, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
+            return;
+        }
+    }
+    focal_strip_impl<CPT, false, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
 }
 
 // One raster: block = 4 waves side by side, one row strip per block.
@@ -555,19 +568,19 @@ __global__ __launch_bounds__(256) void focal_batch_kernel(const FocalParams *__r
 // sequence can be captured in a HIP graph and replayed).
 __global__ void focal_queue_reset(unsigned *queue) { queue[0] = 0; }
 
-// Recomputes queued cells with the reference-order FP64 code.  If the queue
-// overflowed it scans every raster for special windows instead.  The last
-// block to finish resets the queue for the next call on the stream.
-// `tiles` is NULL for a single raster (then `one` describes it).
+// Recomputes, with the reference-order FP64 code, every window holding Inf
+// or |z| >= 2^40 in the queued strips (width: columns per strip).  If the
+// queue overflowed it scans every raster instead.  The last block to finish
+// resets the queue for the next call on the stream.  `tiles` is NULL for a
+// single raster (then `one` describes it).
 template <bool DO_SLOPE, bool DO_D8>
 __global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams one, const FocalParams *__restrict__ tiles,
-                                                          int ntiles)
+                                                          int ntiles, int width)
 {
     const unsigned *cnt = one.fix_count;
     const unsigned n = __atomic_load_n(&cnt[0], __ATOMIC_RELAXED);
-    const unsigned stride = gridDim.x * blockDim.x;
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    auto fix = [&](const FocalParams &p, int r, int c, bool check) {
+    auto fix = [&](const FocalParams &p, int r, int c) {
+        if (r < 1 || r > p.rows - 2 || c < 1 || c > p.cols - 2) return;
         double d[9];
         bool nd = false, nd8 = false, sp = false;
         for (int k = 0; k < 9; k++) {
@@ -577,7 +590,7 @@ __global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams one, const
             sp |= !(__builtin_fabsf(f) < 0x1p40f);
             d[k] = (double)f;
         }
-        if (check && !sp) return;
+        if (!sp) return;
         const long long o = (long long)r * p.pitch + c;
         if (DO_D8 && !nd8) {
             unsigned v = d8_exact(d);
@@ -587,17 +600,19 @@ __global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams one, const
         if (DO_SLOPE && !nd) p.slope[o] = slope_unit((float)slope_generic(d, p.method, p.cs), p.unit);
     };
     if (n <= one.fix_cap) {
-        for (unsigned k = tid; k < n; k += stride) {
-            const uint2 e = one.fix_list[k];
+        for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+            const uint4 e = one.fix_list[k];
             const FocalParams &p = tiles ? tiles[e.x] : one;
-            fix(p, (int)(e.y / (unsigned)p.cols), (int)(e.y % (unsigned)p.cols), false);
+            const int r0 = (int)e.y, nr = (int)e.z - (int)e.y, c0 = (int)e.w;
+            for (int t = threadIdx.x; t < nr * width; t += blockDim.x) fix(p, r0 + t / width, c0 + t % width);
         }
     } else {
+        const unsigned stride = gridDim.x * blockDim.x;
+        const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
         for (int t = 0; t < (tiles ? ntiles : 1); t++) {
             const FocalParams &p = tiles ? tiles[t] : one;
             const long long cells = (long long)(p.rows - 2) * (p.cols - 2);
-            for (long long k = tid; k < cells; k += stride)
-                fix(p, 1 + (int)(k / (p.cols - 2)), 1 + (int)(k % (p.cols - 2)), true);
+            for (long long k = tid; k < cells; k += stride) fix(p, 1 + (int)(k / (p.cols - 2)), 1 + (int)(k % (p.cols - 2)));
         }
     }
     __syncthreads();
@@ -614,7 +629,7 @@ __global__ __launch_bounds__(256) void focal_fixup_kernel(FocalParams one, const
 }
 
 constexpr int kFixupBlocks = 256;
-constexpr unsigned kFixupCap = 1u << 16;
+constexpr unsigned kFixupCap = 1u << 14;  // queued strips
 
 struct LaunchCfg {
     int cpt;
@@ -638,12 +653,13 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
 }
 
 static void launch_fixup(const FocalParams &p, const FocalParams *tiles, int ntiles, bool do_slope, bool do_d8,
-                         hipStream_t s)
+                         int cpt, hipStream_t s)
 {
     dim3 block(256), grid(kFixupBlocks);
-    if (do_slope && do_d8) hipLaunchKernelGGL((focal_fixup_kernel<true, true>), grid, block, 0, s, p, tiles, ntiles);
-    else if (do_slope) hipLaunchKernelGGL((focal_fixup_kernel<true, false>), grid, block, 0, s, p, tiles, ntiles);
-    else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p, tiles, ntiles);
+    const int width = 64 * cpt;
+    if (do_slope && do_d8) hipLaunchKernelGGL((focal_fixup_kernel<true, true>), grid, block, 0, s, p, tiles, ntiles, width);
+    else if (do_slope) hipLaunchKernelGGL((focal_fixup_kernel<true, false>), grid, block, 0, s, p, tiles, ntiles, width);
+    else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p, tiles, ntiles, width);
 }
 
 static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
@@ -680,12 +696,12 @@ static LaunchCfg default_cfg(int rows)
     return c;
 }
 
-// The fixup queue: two counters and kFixupCap cell indices, zeroed once at
+// The fixup queue: two counters and kFixupCap strip entries, zeroed once at
 // allocation; every fixup launch leaves the counters at zero again.
 static int ensure_fixup(pargo_ctx *ctx)
 {
     if (ctx->fix_mem) return PARGO_OK;
-    const size_t bytes = 64 * sizeof(unsigned) + (size_t)kFixupCap * sizeof(uint2);
+    const size_t bytes = 64 * sizeof(unsigned) + (size_t)kFixupCap * sizeof(uint4);
     hipError_t e = hipMalloc(&ctx->fix_mem, bytes);
     if (e != hipSuccess) return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the fixup queue: %s", bytes,
                                      hipGetErrorString(e));
@@ -698,7 +714,7 @@ static int ensure_fixup(pargo_ctx *ctx)
 static void attach_fixup(pargo_ctx *ctx, FocalParams &p)
 {
     p.fix_count = (unsigned *)ctx->fix_mem;
-    p.fix_list = (uint2 *)((unsigned *)ctx->fix_mem + 64);
+    p.fix_list = (uint4 *)((unsigned *)ctx->fix_mem + 64);
     p.fix_cap = kFixupCap;
 }
 
@@ -778,7 +794,7 @@ static int run_focal(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int 
     launch_focal(p, cfg, vec, do_slope, do_d8, ctx->stream);
     rc = check_launch(ctx, name);
     if (rc) return rc;
-    launch_fixup(p, nullptr, 1, do_slope, do_d8, ctx->stream);
+    launch_fixup(p, nullptr, 1, do_slope, do_d8, cfg.cpt, ctx->stream);
     return check_launch(ctx, name);
 }
 
@@ -859,7 +875,7 @@ static void enqueue_batch(const BatchPlan &b, hipStream_t s)
     hipLaunchKernelGGL(focal_queue_reset, dim3(1), dim3(1), 0, s, b.d_queue);
     if (b.cpt == 2) launch_batch_c<2>(b, s);
     else launch_batch_c<4>(b, s);
-    launch_fixup(b.h_tiles[0], b.d_tiles, b.ntiles, b.do_slope, b.do_d8, s);
+    launch_fixup(b.h_tiles[0], b.d_tiles, b.ntiles, b.do_slope, b.do_d8, b.cpt, s);
 }
 
 static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype, int cpt,
@@ -963,7 +979,7 @@ static int run_batch(pargo_ctx *ctx, BatchPlan *b, int mode, int rows_per_strip)
         for (int t = 0; t < b->ntiles; t++) {
             const FocalParams &p = b->h_tiles[t];
             launch_focal(p, cfg, p.vec != 0, b->do_slope, b->do_d8, s);
-            launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, s);
+            launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, b->cpt, s);
         }
         return check_launch(ctx, "pargo_batch_slope_d8 (per tile)");
     }
