@@ -127,6 +127,14 @@ def secondary_idw(torch, synth, IDWOperator, with_cpu):
            "roofline": {"bound": "fp64 valu", "achieved": round(flop / ms / 1e9, 3), "peak": FP64_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(flop / ms / 1e9 / FP64_PEAK_TFLOPS, 4),
                         "flop_per_launch": flop}}
+    # the IDW adjoint (dL/d sample values) for an L2 loss against a seeded
+    # target grid: dL/d(out) = 2 (out - target)
+    gen = torch.Generator(device="cuda").manual_seed(synth.BASE_SEED + 3)
+    target = out + torch.randn(out.shape, device="cuda", generator=gen)
+    grad = (2.0 * (out - target)).contiguous()
+    ams = event_time(torch, lambda: op.adjoint(grad), 3, warm=1)
+    res["adjoint_ms"] = round(ams, 4)
+    res["adjoint_fwd_ratio"] = round(ams / ms, 3)
     if with_cpu:
         from oracle import pyoracle
         import numpy as np
@@ -167,8 +175,10 @@ def secondary_batch(torch, batch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50)
+    # the first tens of launches run slower (clock and power ramp-up); the
+    # default warm-up covers them
+    ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--cpu-rows", type=int, default=4096, help="rows of the DEM in the CPU baseline sample")
     ap.add_argument("--cpu-repeats", type=int, default=3)
@@ -214,25 +224,26 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    # Timed region: K steps, HIP events around every launch on the stream the
-    # kernels run on (torch's current stream, which the context is bound to).
+    # Timed region: K back-to-back steps bracketed by synchronisation, with
+    # HIP events at its two ends on the stream the kernels run on (torch's
+    # current stream, which the context is bound to).  An event between
+    # every step would add a timestamp packet and a cache flush to each one.
     stream = torch.cuda.current_stream(dev)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    s_ev, e_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
-    for s_ev, e_ev in evs:
-        s_ev.record(stream)
+    s_ev.record(stream)
+    for _ in range(a.steps):
         step()
-        e_ev.record(stream)
+    e_ev.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    kern_ms = [s.elapsed_time(e) for s, e in evs]
-    avg_ms = sum(kern_ms) / len(kern_ms)
+    avg_ms = s_ev.elapsed_time(e_ev) / a.steps
     ms_per_step = wall * 1000.0 / a.steps
     cells = a.size * a.size
     value = cells / (ms_per_step / 1000.0) / 1e9
     achieved = BYTES_PER_CELL * cells / (avg_ms / 1000.0) / 1e9
-    log("step avg %.4f ms (min %.4f max %.4f), wall %.4f ms/step" % (avg_ms, min(kern_ms), max(kern_ms), ms_per_step))
+    log("step avg %.4f ms (events), wall %.4f ms/step" % (avg_ms, ms_per_step))
 
     out = {
         "metric": "Gcells/s slope+D8 on 16384^2 DEM, % HBM roofline; adjoint/fwd time ratio",

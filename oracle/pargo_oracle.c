@@ -738,6 +738,237 @@ int oracle_val_row_dcmp(const double *load, int rows, int cols, double noData, i
 }
 
 /* ------------------------------------------------------------------ */
+/* D8 contributing area (apps/hydrology/scad8/scaOperator.cpp:5-131)    */
+/* run by RasterOperator::Work's MPI_Type loop at one rank              */
+/* (gpro/rasterOperator.h:147-216).                                     */
+/* ------------------------------------------------------------------ */
+#define SCA_EPS 0.000001 /* scaOperator.cpp:3 */
+
+typedef struct {
+    cellspace d8, sca, degree; /* degree is a RasterLayer<double> (scaOperator.h:37) */
+    double noData;
+    int num, maxRow, maxCol;
+    int termination;
+    int converge; /* 0: the reference; 1: phase-2 sweeps until nothing changes */
+    int64_t changes;
+} sca_op;
+
+static int sca_cell(void *vself, int iRow, int iCol)
+{
+    sca_op *o = (sca_op *)vself;
+    double *d8c = cs_row(&o->d8, iRow), *scac = cs_row(&o->sca, iRow), *degc = cs_row(&o->degree, iRow);
+    if (!d8c || !scac || !degc) return 0;
+    const int last = (iRow == o->maxRow && iCol == o->maxCol);
+
+    if (o->num == 0) { /* scaOperator.cpp:46-63 */
+        if (fabs(d8c[iCol] - o->noData) < SCA_EPS) {
+            degc[iCol] = -2;
+            scac[iCol] = o->noData;
+        } else {
+            degc[iCol] = 0;
+            scac[iCol] = 1;
+        }
+        if (last) {
+            o->num = 1;
+            o->termination = 0;
+        }
+        return 1;
+    }
+    if (o->num == 1) { /* scaOperator.cpp:65-84: count the neighbours that drain here */
+        int dir = 8;
+        for (int tRow = iRow - 1; tRow <= iRow + 1; tRow++) {
+            double *d8t = cs_row(&o->d8, tRow);
+            if (!d8t) return 0;
+            for (int tCol = iCol - 1; tCol <= iCol + 1; tCol++) {
+                if (d8t[tCol] == dir && (dir + d8c[iCol] != 8) && dir != 4 &&
+                    fabs(d8c[iCol] - o->noData) > SCA_EPS)
+                    degc[iCol]++;
+                dir--;
+            }
+        }
+        if (last) {
+            o->num = 2;
+            o->termination = 0;
+        }
+        return 1;
+    }
+    /* scaOperator.cpp:86-128: pull finished upstream neighbours.  The last
+     * work cell always runs the loop in the reference; the converging
+     * variant applies the degree test to it like to every other cell. */
+    if (degc[iCol] <= 0 && !(last && !o->converge)) return 1;
+    int dir = 8;
+    for (int tRow = iRow - 1; tRow <= iRow + 1; tRow++) {
+        double *d8t = cs_row(&o->d8, tRow), *scat = cs_row(&o->sca, tRow), *degt = cs_row(&o->degree, tRow);
+        if (!d8t || !scat || !degt) return 0;
+        for (int tCol = iCol - 1; tCol <= iCol + 1; tCol++) {
+            if ((degt[tCol] == 0 || degt[tCol] == -1) && d8t[tCol] == dir) {
+                if ((dir + d8c[iCol] != 8) && dir != 4) {
+                    scac[iCol] += scat[tCol];
+                    degc[iCol]--;
+                    if (degc[iCol] == 0) degc[iCol] = -1;
+                    o->changes++;
+                }
+                if (degt[tCol] == -1) {
+                    degt[tCol] = -2;
+                    o->changes++;
+                }
+            }
+            dir--;
+        }
+    }
+    if (last && !o->converge) {
+        /* end of the sweep (scaOperator.cpp:110-127); Termination stays 1,
+         * so Work leaves its loop after this single accumulation sweep */
+        for (int i = 1; i <= o->maxRow; ++i) {
+            double *degi = cs_row(&o->degree, i);
+            if (!degi) return 0;
+            for (int j = 1; j <= o->maxCol; ++j) {
+                if (degi[j] == 0) degi[j] = -2;
+                else if (degi[j] == -1) degi[j] = 0;
+            }
+        }
+        o->termination = 1;
+    }
+    return 1;
+}
+
+/* Contributing area from a D8 layer of window-index codes (dirType 8,
+ * flowdird8).  The SCA layer and the degree layer start as copies of the D8
+ * layer's metadata filled with its NoData value (rasterLayer.h:739-745), so
+ * border cells come out as NoData.  converge = 0 reproduces the reference:
+ * one initial sweep, one degree sweep and a single accumulation sweep
+ * (every phase-2 sweep ends with Termination = 1, scaOperator.cpp:126).
+ * converge = 1 repeats the accumulation sweep, with the reference's
+ * end-of-sweep degree update, until a sweep changes nothing, and treats
+ * the last work cell like the others.  *sweeps (may be NULL) receives the
+ * number of Work sweeps.  Returns 0. */
+int oracle_sca_d8(const double *d8, int rows, int cols, double noData, int converge,
+                  double *sca_out, int *sweeps)
+{
+    size_t n = (size_t)rows * (size_t)cols;
+    double *deg = (double *)malloc(n * sizeof(double) + 1);
+    if (!deg) return -1;
+    for (size_t i = 0; i < n; i++) {
+        sca_out[i] = noData;
+        deg[i] = noData;
+    }
+    sca_op o;
+    memset(&o, 0, sizeof(o));
+    o.d8.m = (double *)d8; o.d8.rows = rows; o.d8.cols = cols;
+    o.sca.m = sca_out; o.sca.rows = rows; o.sca.cols = cols;
+    o.degree.m = deg; o.degree.rows = rows; o.degree.cols = cols;
+    o.noData = noData;
+    o.maxRow = rows - 2;
+    o.maxCol = cols - 2;
+    o.converge = converge;
+    int nsweep = 0, rc = 0;
+    if (!converge) {
+        do { /* rasterOperator.h:170-216 */
+            o.termination = 1;
+            if (work_loop(sca_cell, &o, 1, rows - 2, 1, cols - 2, 1) != 0) { rc = -1; break; }
+            nsweep++;
+        } while (!o.termination);
+    } else {
+        for (int ph = 0; ph < 2 && rc == 0; ph++) {
+            if (work_loop(sca_cell, &o, 1, rows - 2, 1, cols - 2, 1) != 0) rc = -1;
+            nsweep++;
+        }
+        if (rows < 3 || cols < 3) o.num = 2;
+        while (rc == 0) {
+            o.changes = 0;
+            if (work_loop(sca_cell, &o, 1, rows - 2, 1, cols - 2, 1) != 0) { rc = -1; break; }
+            nsweep++;
+            for (int i = 1; i <= rows - 2; ++i)
+                for (int j = 1; j <= cols - 2; ++j) {
+                    double *dg = deg + (size_t)i * cols + j;
+                    if (*dg == 0) { *dg = -2; o.changes++; }
+                    else if (*dg == -1) { *dg = 0; o.changes++; }
+                }
+            if (o.changes == 0) break;
+        }
+    }
+    free(deg);
+    if (sweeps) *sweeps = nsweep;
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Depression filling (apps/hydrology/pitremove/pitRemoveOperator.cpp:  */
+/* 3-72), Work's MPI_Type loop at one rank.                             */
+/* ------------------------------------------------------------------ */
+#define PIT_EPS 0.000001 /* pitRemoveOperator.h:85 */
+
+typedef struct {
+    cellspace dem, wdem;
+    double noData;
+    int xSize, ySize, nbrRadius, num;
+    int termination;
+} pit_op;
+
+static int pit_cell(void *vself, int iRow, int iCol)
+{
+    pit_op *o = (pit_op *)vself;
+    double *demc = cs_row(&o->dem, iRow), *wc = cs_row(&o->wdem, iRow);
+    if (!demc || !wc) return 0;
+    const double gap = 0.0005; /* pitRemoveOperator.cpp:36 */
+    if (o->num == 0) {
+        if (fabs(demc[iCol] - o->noData) > PIT_EPS)
+            wc[iCol] = 10000.0;
+        else
+            wc[iCol] = o->noData;
+        if (iRow == o->xSize - 2 && iCol == o->ySize - 2) {
+            o->num = 1;
+            o->termination = 0;
+        }
+        return 1;
+    }
+    if (fabs(demc[iCol] - o->noData) <= PIT_EPS || wc[iCol] <= demc[iCol]) return 1;
+    for (int i = iRow - o->nbrRadius; i <= iRow + o->nbrRadius; i++) {
+        double *wi = cs_row(&o->wdem, i);
+        if (!wi) return 0;
+        for (int j = iCol - o->nbrRadius; j <= iCol + o->nbrRadius; j++) {
+            if ((demc[iCol] >= (wi[j] + gap)) || fabs(wi[j] - o->noData) < PIT_EPS) {
+                wc[iCol] = demc[iCol];
+                o->termination = 0;
+            } else if (wc[iCol] > (wi[j] + gap)) {
+                wc[iCol] = wi[j] + gap;
+                o->termination = 0;
+            }
+        }
+    }
+    return 1;
+}
+
+/* Planchon-Darboux style filling with a 0.0005 gap, swept in row-major
+ * order until a sweep assigns nothing.  wdem starts as a copy of the DEM
+ * layer's metadata filled with NoData (rasterLayer.h:739-745), so border
+ * cells stay NoData and act as outlets.  max_sweeps <= 0 means no limit;
+ * returns 1 if the limit stopped the loop, 0 when it converged. */
+int oracle_pit_remove(const double *dem, int rows, int cols, double noData, double *wdem_out,
+                      int max_sweeps, int *sweeps)
+{
+    size_t n = (size_t)rows * (size_t)cols;
+    for (size_t i = 0; i < n; i++) wdem_out[i] = noData;
+    pit_op o;
+    memset(&o, 0, sizeof(o));
+    o.dem.m = (double *)dem; o.dem.rows = rows; o.dem.cols = cols;
+    o.wdem.m = wdem_out; o.wdem.rows = rows; o.wdem.cols = cols;
+    o.noData = noData;
+    o.xSize = rows;
+    o.ySize = cols;
+    o.nbrRadius = 1;
+    int nsweep = 0, rc = 0;
+    do {
+        o.termination = 1;
+        if (work_loop(pit_cell, &o, 1, rows - 2, 1, cols - 2, 1) != 0) { rc = -1; break; }
+        nsweep++;
+        if (max_sweeps > 0 && nsweep >= max_sweeps && !o.termination) { rc = 1; break; }
+    } while (!o.termination);
+    if (sweeps) *sweeps = nsweep;
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
 /* Helpers used by the tests.                                          */
 /* ------------------------------------------------------------------ */
 

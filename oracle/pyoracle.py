@@ -35,6 +35,10 @@ def lib():
                                       ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_val_row_dcmp.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
+        L.oracle_sca_d8.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, P,
+                                    ctypes.POINTER(ctypes.c_int)]
+        L.oracle_pit_remove.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int)]
         L.oracle_div_sqrt2.argtypes = [P, P, ctypes.c_int64]
         L.oracle_smpl1d_dcmp.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2
         L.oracle_num_threads.restype = ctypes.c_int
@@ -140,6 +144,31 @@ def val_row_dcmp(load, nodata, nsub):
     if rc != 0:
         raise ValueError("oracle_val_row_dcmp failed: %d" % rc)
     return first
+
+
+def sca_d8(d8, nodata=-9999.0, converge=False):
+    """Contributing area of a float64 D8 layer (window-index codes); returns
+    (sca float64, number of Work sweeps)."""
+    d = np.ascontiguousarray(d8, dtype=np.float64)
+    rows, cols = d.shape
+    out = np.empty_like(d)
+    sw = ctypes.c_int(0)
+    rc = lib().oracle_sca_d8(_p(d), rows, cols, float(nodata), int(bool(converge)), _p(out), ctypes.byref(sw))
+    if rc != 0:
+        raise RuntimeError("oracle_sca_d8 failed")
+    return out, sw.value
+
+
+def pit_remove(dem, nodata=-9999.0, max_sweeps=0):
+    """Depression filling; returns (filled float64, sweeps, converged)."""
+    z = _widen(dem) if np.asarray(dem).dtype == np.float32 else np.ascontiguousarray(dem, dtype=np.float64)
+    rows, cols = z.shape
+    out = np.empty_like(z)
+    sw = ctypes.c_int(0)
+    rc = lib().oracle_pit_remove(_p(z), rows, cols, float(nodata), _p(out), int(max_sweeps), ctypes.byref(sw))
+    if rc < 0:
+        raise RuntimeError("oracle_pit_remove failed")
+    return out, sw.value, rc == 0
 
 
 def div_sqrt2(x):

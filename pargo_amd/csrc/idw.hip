@@ -156,8 +156,13 @@ __device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, doub
                 const int bi = tRow * p.blockCols + tCol;
                 const int s0 = p.start[bi], s1 = p.start[bi + 1];
                 ncand += (unsigned)(s1 - s0);
+                // one candidate ahead: the load of the next coordinates
+                // overlaps this candidate's distance test (the bucket
+                // array carries one padding entry after the last sample)
+                double2 nxt = p.xy[s0];
                 for (int q = s0; q < s1; q++) {
-                    const double2 xy = p.xy[q];
+                    const double2 xy = nxt;
+                    nxt = p.xy[q + 1];
                     const double dx = xy.x - cellX, dy = xy.y - cellY;
                     const double s = dx * dx + dy * dy;
                     if (s <= kthr) {
@@ -206,10 +211,37 @@ __device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, doub
 }
 
 // ADJOINT: instead of writing the grid, add g(cell) * w_i / W to the
-// gradient of each selected sample (bucket order, FP64 atomics).
+// gradient of each selected sample (bucket order).  The cells of a workgroup
+// select few distinct samples, so their contributions are first summed in
+// an LDS hash table (open addressing on the bucket slot) and flushed with
+// one global FP64 atomic per distinct sample; a crowded table falls back to
+// global atomics.
+constexpr int kAdjTable = 1024;
+constexpr int kAdjProbe = 16;
+
 template <int MAXK, bool ADJOINT>
 __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
 {
+    __shared__ int tkey[ADJOINT ? kAdjTable : 1];
+    __shared__ double tval[ADJOINT ? kAdjTable : 1];
+    if constexpr (ADJOINT) {
+        for (int i = threadIdx.x; i < kAdjTable; i += blockDim.x) {
+            tkey[i] = -1;
+            tval[i] = 0.0;
+        }
+        __syncthreads();
+    }
+    auto adj_add = [&](int q, double v) {
+        unsigned h = ((unsigned)q * 2654435761u) >> 22;  // 10-bit hash
+        for (int t = 0; t < kAdjProbe; t++, h = (h + 1) & (kAdjTable - 1)) {
+            const int k = atomicCAS(&tkey[h], -1, q);
+            if (k == -1 || k == q) {
+                unsafeAtomicAdd(&tval[h], v);
+                return;
+            }
+        }
+        atomicAdd(&p.grad_acc[q], v);
+    };
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int col = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
@@ -244,7 +276,7 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                 const double gc = (double)p.grad_out[(long long)row * p.cols + col];
 #pragma unroll
                 for (int i = 0; i < MAXK; i++)
-                    if (i < num && gc != 0.0) atomicAdd(&p.grad_acc[slot[i]], gc * w[i] / wsum);
+                    if (i < num && gc != 0.0) adj_add(slot[i], gc * w[i] / wsum);
             } else {
                 double acc = 0.0;
 #pragma unroll
@@ -254,6 +286,13 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
             }
         }
         if (!ADJOINT) p.out[(long long)row * p.cols + col] = result;
+    }
+    if constexpr (ADJOINT) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < kAdjTable; i += blockDim.x) {
+            const int q = tkey[i];
+            if (q >= 0) atomicAdd(&p.grad_acc[q], tval[i]);
+        }
     }
     if (p.cand_out) {
         unsigned long long c = ncand;
@@ -455,7 +494,7 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
     }
     for (int b = 0; b < nb; b++) start[b + 1] += start[b];
     std::vector<int> fill(start.begin(), start.end() - 1);
-    std::vector<double2> xy(n);
+    std::vector<double2> xy((size_t)n + 1, make_double2(0.0, 0.0));
     std::vector<double> v(n);
     pl->h_perm.resize(n);
     for (int i = 0; i < n; i++) {
@@ -464,7 +503,7 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
         v[q] = vals[i];
         pl->h_perm[q] = i;
     }
-    const size_t bytes_xy = sizeof(double2) * n, bytes_v = sizeof(double) * n;
+    const size_t bytes_xy = sizeof(double2) * ((size_t)n + 1), bytes_v = sizeof(double) * n;
     const size_t bytes_start = sizeof(int) * (nb + 1), bytes_perm = sizeof(int) * n;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t total = al(bytes_xy) + al(bytes_v) + al(bytes_start) + al(bytes_perm);

@@ -193,3 +193,14 @@ def test_repeatable_bitwise():
     b = focal.slope_d8(t, 30.0, -9999.0)
     torch.cuda.synchronize()
     assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("unit", ["degrees", "tan"])
+def test_slope_1k_recipe(unit):
+    # the 1024 x 1024 recipe DEM (20 NoData discs, about 5% NoData) in both
+    # output units, all methods sharing one upload
+    z = synth.workload_dem("slope1k")
+    frac = float(np.mean(z == -9999.0))
+    print("slope1k NoData fraction %.4f" % frac)
+    assert 0.005 < frac < 0.2
+    check_case(z, unit=unit)
