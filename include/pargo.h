@@ -161,6 +161,48 @@ int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, in
 int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
                          int *cols_per_lane, int *rows_per_strip);
 
+/* ---- D8 contributing area (scad8) ------------------------------------ */
+
+/* Flow classes, one uint8 per cell (what pargo_sca_d8 reads):
+ *   0..8   a valid cell draining to window index k (0=NW 1=N 2=NE 3=W 5=E
+ *          6=SW 7=S 8=SE), the dirType-8 codes of pargo_d8; 4 drains nowhere
+ *   PARGO_FLOW_EXCL | k   a valid cell that drains nowhere but, like a cell
+ *          draining to k, takes no inflow from window cell k (a D8 value v
+ *          with dir + v == 8 in double, dir = 8 - k, that is not exactly k)
+ *   PARGO_FLOW_NODATA     fabs(v - nodata) < 1e-6
+ *   PARGO_FLOW_INERT      neither < nor > 1e-6 from nodata (NaN, or exactly
+ *          1e-6 away): area 1,
+ *          takes no inflow (scaOperator.cpp:48 and :71 disagree on it)
+ *   any other value       a valid cell that drains nowhere (the reference's
+ *          -1 for flats and pits; pargo_d8's 255). */
+#define PARGO_FLOW_EXCL 0x10
+#define PARGO_FLOW_INERT 253
+#define PARGO_FLOW_NODATA 254
+#define PARGO_FLOW_NONE 255
+
+#define PARGO_SCA_REFERENCE 0 /* the reference: a single accumulation sweep */
+#define PARGO_SCA_CONVERGE 1  /* accumulation sweeps repeated until nothing changes */
+
+/* Flow classes of a reference D8 layer (device double, dirType-8 codes,
+ * NoData grid->nodata) with the reference's own comparisons.  NaN NoData,
+ * or NoData within 2e-6 of an integer in -1..8, is refused: the reference
+ * would let such NoData cells drain and count border cells as finished. */
+int pargo_d8_classify(pargo_ctx *ctx, const pargo_grid *grid, const double *d8, uint8_t *flow_out);
+
+/* Specific catchment area in cells of every interior cell: SCAOperator
+ * (apps/hydrology/scad8/scaOperator.cpp:37-131) driven by
+ * RasterOperator::Work (gpro/rasterOperator.h:170-216).  flow holds flow
+ * classes (pargo_d8 dirtype-8 output qualifies); nodata_mask (may be NULL)
+ * marks further NoData cells, e.g. pargo_d8's mask.  sca_out (device double,
+ * grid->pitch) gets NoData on the border and on NoData cells, as the
+ * reference's SCA layer.  PARGO_SCA_REFERENCE reproduces the reference's
+ * result bit for bit, including that it stops after one accumulation sweep
+ * in row-major order and that its last work cell (rows-2, cols-2) adds
+ * inflows to NoData; PARGO_SCA_CONVERGE gives the area that repeated sweeps
+ * converge to (full D8 flow accumulation).  grid->cellsize is unused. */
+int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *grid, const uint8_t *flow, const uint32_t *nodata_mask,
+                 int mode, double *sca_out);
+
 /* ---- IDW ------------------------------------------------------------ */
 
 /* Output grid of IDW for these samples (host arrays, file order), by the

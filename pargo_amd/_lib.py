@@ -22,6 +22,12 @@ UNIT_TAN = 0
 UNIT_DEGREES = 1
 D8_FLAT_256 = 0
 D8_FLAT_8 = 255
+FLOW_EXCL = 0x10
+FLOW_INERT = 253
+FLOW_NODATA = 254
+FLOW_NONE = 255
+SCA_REFERENCE = 0
+SCA_CONVERGE = 1
 
 # Every exported symbol of include/pargo.h with (restype, argtypes).
 _c = ctypes
@@ -57,6 +63,8 @@ _SIGS = {
     "pargo_geotiff_read": (_c.c_int, [_c.c_char_p, _P, _P]),
     "pargo_geotiff_write": (_c.c_int, [_c.c_char_p, _P, _P, _c.c_int, _c.c_char_p]),
     "pargo_points_read": (_c.c_int, [_c.c_char_p, _c.c_int, _c.POINTER(_c.c_int), _P, _P, _P]),
+    "pargo_d8_classify": (_c.c_int, [_P, _P, _P, _P]),
+    "pargo_sca_d8": (_c.c_int, [_P, _P, _P, _P, _c.c_int, _P]),
     "pargo_slope_adjoint": (_c.c_int, [_P, _P, _P, _P, _P]),
     "pargo_idw_adjoint": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
                                      _c.c_int, _P, _P]),

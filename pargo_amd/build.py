@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-ffp-contract=off"]
 
-HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip"]
+HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip", "sca.hip"]
 CXX_SOURCES = ["synth.cpp", "dcmp.cpp", "io.cpp", "../gpro/geotiff.cpp", "../gpro/shapefile.cpp"]
 
 
@@ -70,12 +70,13 @@ def build_library(jobs=8):
 
 GPRO = os.path.join(PKG, "gpro")
 # the GeoTIFF and shapefile codecs are part of libpargo.so (pargo_geotiff_*)
-GPRO_SOURCES = ["application.cpp", "focalDevice.cpp"]
+GPRO_SOURCES = ["application.cpp", "focalDevice.cpp", "hydroDevice.cpp"]
 # command line -> (app directory, sources)
 APP_SOURCES = {
     "slope": ("slope", ["slopeOperator.cpp", "pSlope.cpp"]),
     "flowdird8": ("flowdird8", ["d8Operator.cpp", "pD8.cpp"]),
     "idw": ("idw", ["idwOperator.cpp", "idwTransformation.cpp", "pIDW.cpp"]),
+    "scad8": ("scad8", ["scaOperator.cpp", "pSCA.cpp"]),
     # host-only checks of the gpro layer, run by tests/test_gpro_host.py
     "gpro_host_test": ("../../tests/cpp", ["gpro_host_test.cpp"]),
 }

@@ -147,3 +147,23 @@ def test_idw_cli_refuses_timing_load(tmp_path):
     p = run([os.path.join(BIN, "idw"), "-sample", SAMPLES, "-out", str(tmp_path / "x.tif")] + IDW_ARGS +
             ["-dcmp", "space", "-writeLoad", str(tmp_path / "t.tif")])
     assert p.returncode != 0 and "per-cell" in p.stdout
+
+
+@pytest.mark.parametrize("converge", [False, True])
+def test_scad8_cli_chain(tmp_path, converge):
+    """flowdird8 (dirType 8) then scad8 on dem.tif, against the oracle's
+    D8 layer and its SCA restatement."""
+    d8f, out = str(tmp_path / "d8.tif"), str(tmp_path / "sca.tif")
+    run_ok([os.path.join(BIN, "flowdird8"), DEM, MOORE, d8f, "8"])
+    run_ok([os.path.join(BIN, "scad8"), d8f, MOORE, out] + (["-converge"] if converge else []))
+    got, info = io.read_geotiff(out)
+    z, dinfo = dem_f32()
+    ref, _ = po.sca_d8(po.d8(z, nodata=-9999.0, dirtype=8), -9999.0, converge)
+    assert info["transform"] == dinfo["transform"] and info["nodata"] == -9999.0
+    assert np.array_equal(got, ref)
+    assert got.max() > 100  # dem.tif has real drainage
+
+
+def test_scad8_cli_wrong_arguments(tmp_path):
+    p = run([os.path.join(BIN, "scad8"), DEM, MOORE])
+    assert p.returncode == 0 and "please input right parameter" in p.stdout
