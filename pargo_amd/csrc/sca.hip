@@ -85,14 +85,20 @@ __device__ __forceinline__ unsigned load_class(const ScaParams &p, int r, int c)
 }
 
 // Static words of a 64 x 8 tile.  Classes of the tile plus a 2-cell halo
-// go to LDS; the inflow count of the tile plus a 1-cell halo follows
-// (a cell needs to know which of its inflows are sources).
-__global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p)
+// go to LDS, then the inflow mask of the tile plus a 1-cell halo (bit j set
+// when the neighbour at window position j < 4 ? j : j + 1 drains into the
+// cell and is not excluded; a cell needs to know which of its inflows are
+// sources, and whether its target counts it).
+__device__ __forceinline__ int win_of_bit(int j) { return j < 4 ? j : j + 1; }
+__device__ __forceinline__ int bit_of_win(int k) { return k < 4 ? k : k - 1; }
+
+__global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, unsigned long long *cursor)
 {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *cursor = 0;  // for the walk kernel
     constexpr int CW = kTileC + 4, CH = kTileR + 4;  // classes, halo 2
-    constexpr int NW = kTileC + 2, NH = kTileR + 2;  // inflow counts, halo 1
+    constexpr int NW = kTileC + 2, NH = kTileR + 2;  // inflow masks, halo 1
     __shared__ uint8_t cls[CH][CW];
-    __shared__ uint8_t nin[NH][NW];
+    __shared__ uint8_t inm[NH][NW];
     const int tid = threadIdx.x;
     const int r0 = blockIdx.y * kTileR, c0 = blockIdx.x * kTileC;
     for (int i = tid; i < CH * CW; i += 256) {
@@ -103,17 +109,17 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p)
     for (int i = tid; i < NH * NW; i += 256) {
         const int lr = i / NW + 1, lc = i % NW + 1;  // position in cls
         const unsigned b = cls[lr][lc];
-        unsigned n = 0;
+        unsigned m = 0;
         if (is_valid(b)) {
             const unsigned ex = excl_of(b);
 #pragma unroll
-            for (int k = 0; k < 9; k++) {
-                if (k == 4) continue;
+            for (int j = 0; j < 8; j++) {
+                const int k = win_of_bit(j);
                 const unsigned u = cls[lr + k / 3 - 1][lc + k % 3 - 1];
-                n += (flow_of(u) == (unsigned)(8 - k) && ex != (unsigned)k) ? 1u : 0u;
+                m |= (flow_of(u) == (unsigned)(8 - k) && ex != (unsigned)k) ? 1u << j : 0u;
             }
         }
-        nin[lr - 1][lc - 1] = (uint8_t)n;
+        inm[lr - 1][lc - 1] = (uint8_t)m;
     }
     __syncthreads();
     for (int i = tid; i < kTileR * kTileC; i += 256) {
@@ -126,27 +132,25 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p)
         if (interior && b == PARGO_FLOW_INERT) {
             word = 1ull << 62;
         } else if (interior && b != PARGO_FLOW_NODATA) {
-            const unsigned ex = excl_of(b);
+            const unsigned mine = inm[tr + 1][tc + 1];
             unsigned npend = 0, nsrc = 0, poison = 0;
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                if (k == 4) continue;
+            for (unsigned m = mine; m != 0; m &= m - 1) {
+                const int k = win_of_bit(__builtin_ctz(m));
                 const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
-                const unsigned u = cls[tr + 2 + k / 3 - 1][tc + 2 + k % 3 - 1];
-                if (flow_of(u) != (unsigned)(8 - k) || ex == (unsigned)k) continue;
                 const bool u_interior = ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2;
                 if (!u_interior) poison = 1;
-                else if (nin[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0) nsrc++;
+                else if (inm[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0) nsrc++;
                 else if (p.converge || k < 4) npend++;
                 else poison = 1;
             }
             const unsigned f = flow_of(b);
             unsigned feeds = 0;
-            if (f != kNone && nin[tr + 1][tc + 1] != 0 && (p.converge || f > 4)) {
+            if (f != kNone && mine != 0 && (p.converge || f > 4)) {
+                // the target counts this cell when its inflow mask has it
                 const int trr = r + (int)f / 3 - 1, tcc = c + (int)f % 3 - 1;
                 const bool t_interior = trr >= 1 && trr <= p.rows - 2 && tcc >= 1 && tcc <= p.cols - 2;
-                const unsigned tb = cls[tr + 2 + (int)f / 3 - 1][tc + 2 + (int)f % 3 - 1];
-                feeds = (t_interior && is_valid(tb) && excl_of(tb) != 8u - f) ? 1u : 0u;
+                const unsigned tm = inm[tr + 1 + (int)f / 3 - 1][tc + 1 + (int)f % 3 - 1];
+                feeds = (t_interior && ((tm >> bit_of_win(8 - (int)f)) & 1u)) ? 1u : 0u;
             }
             word = ((unsigned long long)npend << 48) | ((unsigned long long)nsrc << 52) |
                    ((unsigned long long)poison << 56) | ((unsigned long long)feeds << 57) |
@@ -162,31 +166,108 @@ __device__ __forceinline__ bool poisoned_of(unsigned long long w) { return (w >>
 __device__ __forceinline__ bool feeds_of(unsigned long long w) { return (w >> 57) & 1ull; }
 __device__ __forceinline__ int code_of(unsigned long long w) { return (int)(w >> 58) & 15; }
 
-// One thread per cell; cells that wait for nothing and feed their target
-// start a walk down the chain of completions.
-__global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p)
+// Walks, persistent: every lane runs one hop per iteration; a lane without
+// a walk takes the next cell of its wave's chunk (chunks of kChunk cells come
+// from a global cursor) and starts a walk there if the cell waits for
+// nothing and feeds its target.  Lanes stay busy until the cursor passes the
+// last cell and every walk of the wave has ended.  Checking one cell per
+// idle lane and iteration keeps the number of hops in flight, and so the
+// queueing delay on the longest chain, moderate: starting every walk at
+// once was slower.
+//
+// A hop first reads the target's word.  A target with one pending inflow
+// has this walk as its only writer, so the walk completes it with a plain
+// store; only confluences (two or more pending inflows, a few percent of
+// the hops on D8 grids) take the atomicAdd.
+constexpr long long kChunk = 4096;
+
+__device__ __forceinline__ long long first_lane(long long v)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
-    if (c >= p.cols) return;
-    long long cur = (long long)r * p.pitch + c;
-    unsigned long long st = p.w[cur];
-    if (!((st >> 62) & 1ull) || npend_of(st) != 0 || !feeds_of(st)) return;
-    unsigned long long val = 1ull + nsrc_of(st);
-    bool done = !poisoned_of(st);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffll));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+__global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p, unsigned long long *cursor)
+{
+    const long long ncells = (long long)p.rows * p.cols;
+    long long chunk_next = 0, chunk_end = 0;  // the same in every lane of a wave
+    bool more = true;                         // the same in every lane of a wave
+    long long cur = -1;                       // cell this lane's walk stands on
+    int code = 0;
+    unsigned long long val = 0;
+    bool done = false;
     for (;;) {
-        const int f = code_of(st);
-        const long long t = cur + (long long)(f / 3 - 1) * p.pitch + (f % 3 - 1);
-        const unsigned long long add = (done ? val : 0ull) | ((unsigned long long)(!done) << 40) | (1ull << 44);
-        const unsigned long long old = atomicAdd(&p.w[t], add);
-        const unsigned arrivals = ((unsigned)(old >> 44) & 15u) + 1u;
-        if (arrivals != npend_of(old)) break;  // an inflow is still on its way
-        const unsigned long long acc = (old & kAccMask) + (done ? val : 0ull);
-        const unsigned not_done = ((unsigned)(old >> 40) & 15u) + (done ? 0u : 1u);
-        done = !poisoned_of(old) && not_done == 0;
-        val = 1ull + nsrc_of(old) + acc;
-        if (!feeds_of(old)) break;
-        cur = t;
-        st = old;
+        // the target's word is requested first, so that its latency
+        // overlaps the refill below
+        long long t = -1;
+        unsigned long long tw = 0;
+        if (cur >= 0) {
+            t = cur + (long long)(code / 3 - 1) * p.pitch + (code % 3 - 1);
+            tw = p.w[t];
+        }
+        const unsigned long long need = __ballot(cur < 0);
+        long long fresh = -1;
+        unsigned long long fw = 0;
+        if (need != 0 && more) {
+            if (chunk_next >= chunk_end) {
+                long long b = 0;
+                if (__lane_id() == 0) b = (long long)atomicAdd(cursor, (unsigned long long)kChunk);
+                b = first_lane(b);
+                if (b >= ncells) {
+                    more = false;
+                } else {
+                    chunk_next = b;
+                    chunk_end = b + kChunk < ncells ? b + kChunk : ncells;
+                }
+            }
+            if (more) {
+                const unsigned rank =
+                    __builtin_amdgcn_mbcnt_hi((unsigned)(need >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)need, 0u));
+                const long long npop = __popcll(need);
+                const long long take = npop < chunk_end - chunk_next ? npop : chunk_end - chunk_next;
+                if (cur < 0 && (long long)rank < take) {
+                    const long long idx = chunk_next + rank;
+                    const int r = (int)(idx / p.cols), c = (int)(idx - (long long)r * p.cols);
+                    fresh = (long long)r * p.pitch + c;
+                    fw = p.w[fresh];
+                }
+                chunk_next += take;
+            }
+        }
+        if (cur >= 0) {
+            const unsigned long long add = (done ? val : 0ull) | ((unsigned long long)(!done) << 40) | (1ull << 44);
+            unsigned long long old;
+            bool complete;
+            if (npend_of(tw) == 1) {
+                old = tw;  // nobody else writes this word
+                p.w[t] = tw + add;
+                complete = true;
+            } else {
+                old = atomicAdd(&p.w[t], add);
+                complete = ((unsigned)(old >> 44) & 15u) + 1u == npend_of(old);
+            }
+            if (!complete) {
+                cur = -1;  // an inflow is still on its way
+            } else {
+                const unsigned long long acc = (old & kAccMask) + (done ? val : 0ull);
+                const unsigned not_done = ((unsigned)(old >> 40) & 15u) + (done ? 0u : 1u);
+                done = !poisoned_of(old) && not_done == 0;
+                val = 1ull + nsrc_of(old) + acc;
+                if (feeds_of(old)) {
+                    cur = t;
+                    code = code_of(old);
+                } else {
+                    cur = -1;
+                }
+            }
+        } else if (fresh >= 0 && ((fw >> 62) & 1ull) && npend_of(fw) == 0 && feeds_of(fw)) {
+            cur = fresh;
+            code = code_of(fw);
+            val = 1ull + nsrc_of(fw);
+            done = !poisoned_of(fw);
+        }
+        if (!more && __ballot(cur >= 0) == 0) break;
     }
 }
 
@@ -318,8 +399,10 @@ extern "C" int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *g, const uint8_t *
     // The final kernel's read of the slot is guarded per cell, but the
     // compiler issues it as a scalar load that runs whatever the lanes
     // decide, so the slot must be a valid address in every mode.
-    rc = pargo::ensure_scratch(ctx, sizeof(double));
+    rc = pargo::ensure_scratch(ctx, 2 * sizeof(double));
     if (rc) return rc;
+    double *slot = static_cast<double *>(ctx->scratch);
+    unsigned long long *cursor = reinterpret_cast<unsigned long long *>(slot + 1);
     pargo::ScaParams p{};
     p.flow = flow;
     p.mask = nodata_mask;
@@ -330,10 +413,11 @@ extern "C" int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *g, const uint8_t *
     p.mask_words = (g->cols + 31) / 32;
     p.converge = mode == PARGO_SCA_CONVERGE;
     dim3 tiles((g->cols + pargo::kTileC - 1) / pargo::kTileC, (g->rows + pargo::kTileR - 1) / pargo::kTileR);
-    hipLaunchKernelGGL(pargo::sca_init_kernel, tiles, dim3(256), 0, ctx->stream, p);
+    hipLaunchKernelGGL(pargo::sca_init_kernel, tiles, dim3(256), 0, ctx->stream, p, cursor);
+    // persistent walk: 8 workgroups of 4 waves per CU
+    const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
+    hipLaunchKernelGGL(pargo::sca_walk_kernel, dim3(cus * 8), dim3(256), 0, ctx->stream, p, cursor);
     dim3 rowsgrid((g->cols + 255) / 256, g->rows);
-    hipLaunchKernelGGL(pargo::sca_walk_kernel, rowsgrid, dim3(256), 0, ctx->stream, p);
-    double *slot = static_cast<double *>(ctx->scratch);
     if (last_rule) hipLaunchKernelGGL(pargo::sca_last_cell_kernel, dim3(1), dim3(1), 0, ctx->stream, p, g->nodata, slot);
     hipLaunchKernelGGL(pargo::sca_final_kernel, rowsgrid, dim3(256), 0, ctx->stream, p, g->nodata, slot);
     return pargo::check_launch(ctx, "pargo_sca_d8");
