@@ -203,6 +203,23 @@ int pargo_d8_classify(pargo_ctx *ctx, const pargo_grid *grid, const double *d8, 
 int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *grid, const uint8_t *flow, const uint32_t *nodata_mask,
                  int mode, double *sca_out);
 
+/* ---- depression filling (pitremove) ------------------------------------ */
+
+/* Filled DEM: PitRemoveOperator (apps/hydrology/pitremove/
+ * pitRemoveOperator.cpp:26-72) driven by RasterOperator::Work
+ * (gpro/rasterOperator.h:170-216) until a sweep changes nothing.  dem is a
+ * device float32 raster (grid->pitch), filled_out a device double raster of
+ * the same pitch: valid interior cells get the reference's value bit for
+ * bit, the border and NoData cells (fabs(dem - nodata) <= 1e-6) get NoData.
+ * The device applies the same updates in another order, which reaches the
+ * same result when no filled value can come within 1e-6 of NoData; a DEM
+ * whose valid elevations reach down to NoData (NoData inside
+ * [min(dem, 10000), 10000]) is refused with PARGO_E_UNSUPPORTED, as is a
+ * DEM holding NaN (the reference never settles next to NoData there).
+ * Synchronous.  *passes (host, may be NULL) receives the number of device
+ * passes. */
+int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *grid, const float *dem, double *filled_out, int *passes);
+
 /* ---- IDW ------------------------------------------------------------ */
 
 /* Output grid of IDW for these samples (host arrays, file order), by the

@@ -65,6 +65,7 @@ _SIGS = {
     "pargo_points_read": (_c.c_int, [_c.c_char_p, _c.c_int, _c.POINTER(_c.c_int), _P, _P, _P]),
     "pargo_d8_classify": (_c.c_int, [_P, _P, _P, _P]),
     "pargo_sca_d8": (_c.c_int, [_P, _P, _P, _P, _c.c_int, _P]),
+    "pargo_pit_remove": (_c.c_int, [_P, _P, _P, _P, _c.POINTER(_c.c_int)]),
     "pargo_slope_adjoint": (_c.c_int, [_P, _P, _P, _P, _P]),
     "pargo_idw_adjoint": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
                                      _c.c_int, _P, _P]),

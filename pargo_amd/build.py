@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-ffp-contract=off"]
 
-HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip", "sca.hip"]
+HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip", "sca.hip", "fill.hip"]
 CXX_SOURCES = ["synth.cpp", "dcmp.cpp", "io.cpp", "../gpro/geotiff.cpp", "../gpro/shapefile.cpp"]
 
 
@@ -77,6 +77,7 @@ APP_SOURCES = {
     "flowdird8": ("flowdird8", ["d8Operator.cpp", "pD8.cpp"]),
     "idw": ("idw", ["idwOperator.cpp", "idwTransformation.cpp", "pIDW.cpp"]),
     "scad8": ("scad8", ["scaOperator.cpp", "pSCA.cpp"]),
+    "pitremove": ("pitremove", ["pitRemoveOperator.cpp", "pPitRemove.cpp"]),
     # host-only checks of the gpro layer, run by tests/test_gpro_host.py
     "gpro_host_test": ("../../tests/cpp", ["gpro_host_test.cpp"]),
 }

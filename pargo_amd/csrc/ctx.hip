@@ -84,9 +84,12 @@ static float key_float(uint32_t k)
     return f;
 }
 
-void nodata_interval(double nodata, double eps, float *lo, float *hi)
+void nodata_interval(double nodata, double eps, float *lo, float *hi, bool inclusive)
 {
-    auto pass = [&](float f) { return std::isfinite(f) && std::fabs((double)f - nodata) < eps; };
+    auto pass = [&](float f) {
+        const double d = std::fabs((double)f - nodata);
+        return std::isfinite(f) && (inclusive ? d <= eps : d < eps);
+    };
     *lo = 1.0f;
     *hi = 0.0f;  // empty
     if (!std::isfinite(nodata) || !(eps > 0)) return;

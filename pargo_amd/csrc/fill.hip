@@ -1,0 +1,290 @@
+// fill.hip -- depression filling (pitremove) on the device.
+//
+// The reference (apps/hydrology/pitremove/pitRemoveOperator.cpp:26-72, run
+// by RasterOperator::Work, gpro/rasterOperator.h:170-216) starts from
+// W = 10000 on valid cells and NoData elsewhere (the border too, through
+// copyLayerInfo) and sweeps the interior in row-major order until a sweep
+// assigns nothing.  A valid cell with W > dem looks at its 3x3 window:
+//   if dem >= W(n) + 0.0005 or W(n) is NoData for some n:  W = dem
+//   otherwise                                              W = min(W, min_n W(n) + 0.0005)
+// (the window loop never breaks, and once W = dem no later neighbour
+// changes it again).
+//
+// That update is monotone in the neighbours' values as long as no valid
+// cell's W can come within 1e-6 of NoData (W stays in [min(dem, 10000),
+// 10000]; pargo_pit_remove checks this and refuses other inputs).  The
+// reference's loop then ends at the greatest fixed point below the start,
+// and so does any order of the same updates: every update reads values that
+// are >= that fixed point, and a state in which no update changes anything
+// is a fixed point.  The result is therefore bit-identical to the
+// reference's while the order is free.
+//
+// Device order: 64 x 32 tiles.  A workgroup loads its tile and a one-cell
+// halo of W into LDS and repeats the update over the tile until nothing in
+// it changes; then it writes the tile back and, for each changed edge,
+// schedules the neighbouring tiles for the next pass.  Passes repeat until
+// one changes nothing.  Tiles read each other's edges while they change;
+// that only feeds the monotone iteration earlier values, never wrong ones.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace pargo {
+
+namespace {
+
+constexpr int kTW = 64, kTH = 32;  // tile: 64 columns x 32 rows, 8 cells per thread
+constexpr double kGap = 0.0005;    // pitRemoveOperator.cpp:36
+constexpr double kStart = 10000.0; // pitRemoveOperator.cpp:41
+
+struct FillParams {
+    const float *__restrict__ dem;
+    double *__restrict__ w;
+    int *__restrict__ stamp;         // per tile: the pass that must process it
+    unsigned *__restrict__ changed;  // per pass of the current batch: tiles that changed
+    long long pitch;
+    int rows, cols, tiles_x, tiles_y;
+    float nd_lo, nd_hi;  // float32 values with fabs((double)v - nodata) <= 1e-6
+    double nodata;
+};
+
+__device__ __forceinline__ bool is_nd(const FillParams &p, float v) { return v >= p.nd_lo && v <= p.nd_hi; }
+
+// W before the first sweep (pitRemoveOperator.cpp:39-45 plus copyLayerInfo).
+__global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+    if (c >= p.cols) return;
+    const long long i = (long long)r * p.pitch + c;
+    const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
+    p.w[i] = (interior && !is_nd(p, p.dem[i])) ? kStart : p.nodata;
+}
+
+__global__ void fill_stamp_init_kernel(int *stamp, int n, unsigned *changed, int npass)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) stamp[i] = 0;
+    if (i < npass) changed[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, int batch_first)
+{
+    const int tile = blockIdx.x;
+    __shared__ double sw[kTH + 2][kTW + 2];
+    __shared__ int edge_flags, run;
+    const int tid = threadIdx.x;
+    // one read of the stamp for the whole workgroup: a neighbour may rewrite
+    // it meanwhile (to pass + 1, which defers this tile by one pass)
+    if (tid == 0) {
+        edge_flags = 0;
+        run = p.stamp[tile] == pass;
+    }
+    __syncthreads();
+    if (!run) return;  // nothing around it changed since it settled
+    const int ty = tile / p.tiles_x, tx = tile - ty * p.tiles_x;
+    const int r0 = ty * kTH, c0 = tx * kTW;
+    for (int i = tid; i < (kTH + 2) * (kTW + 2); i += 256) {
+        const int lr = i / (kTW + 2), lc = i % (kTW + 2);
+        const int r = r0 - 1 + lr, c = c0 - 1 + lc;
+        sw[lr][lc] = (r >= 0 && r < p.rows && c >= 0 && c < p.cols) ? p.w[(long long)r * p.pitch + c] : p.nodata;
+    }
+    // own cells: column tid % 64, rows tid / 64 + 4 k
+    const int lc = tid % kTW + 1;
+    double d[kTH / 4];
+    unsigned live = 0;  // bit k: cell (rows tid/64 + 4k) is a valid interior cell
+#pragma unroll
+    for (int k = 0; k < kTH / 4; k++) {
+        const int r = r0 + tid / kTW + 4 * k, c = c0 + lc - 1;
+        d[k] = 0.0;
+        if (r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2) {
+            const float f = p.dem[(long long)r * p.pitch + c];
+            if (!is_nd(p, f)) {
+                d[k] = (double)f;
+                live |= 1u << k;
+            }
+        }
+    }
+    __syncthreads();
+    unsigned touched = 0;  // bit k: the cell changed during this pass
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < kTH / 4; k++) {
+            if (!((live >> k) & 1u)) continue;
+            const int lr = tid / kTW + 4 * k + 1;
+            const double w = sw[lr][lc];
+            if (w <= d[k]) continue;
+            bool to_dem = false;
+            double m = w;
+#pragma unroll
+            for (int n = 0; n < 9; n++) {
+                if (n == 4) continue;
+                const double wn = sw[lr + n / 3 - 1][lc + n % 3 - 1];
+                const double s = wn + kGap;
+                if (d[k] >= s || fabs(wn - p.nodata) < 0.000001) to_dem = true;
+                else if (m > s) m = s;
+            }
+            const double nw = to_dem ? d[k] : m;
+            if (nw != w) {
+                sw[lr][lc] = nw;
+                any = true;
+                touched |= 1u << k;
+            }
+        }
+        if (!__syncthreads_or(any)) break;
+    }
+    unsigned flags = 0;
+#pragma unroll
+    for (int k = 0; k < kTH / 4; k++) {
+        if (!((touched >> k) & 1u)) continue;
+        const int tr = tid / kTW + 4 * k, tc = lc - 1;
+        const int r = r0 + tr, c = c0 + tc;
+        p.w[(long long)r * p.pitch + c] = sw[tr + 1][lc];
+        flags |= (tr == 0 ? 1u : 0u) | (tr == kTH - 1 ? 2u : 0u) | (tc == 0 ? 4u : 0u) | (tc == kTW - 1 ? 8u : 0u) | 16u;
+    }
+    if (flags) atomicOr(&edge_flags, (int)flags);
+    __syncthreads();
+    if (tid == 0 && edge_flags) {
+        const int f = edge_flags;
+        atomicAdd(&p.changed[pass - batch_first], 1u);
+        const int next = pass + 1;
+        for (int dy = -1; dy <= 1; dy++) {
+            for (int dx = -1; dx <= 1; dx++) {
+                if (dx == 0 && dy == 0) continue;
+                const bool need = (dy == -1 && (f & 1)) || (dy == 1 && (f & 2)) || (dx == -1 && (f & 4)) ||
+                                  (dx == 1 && (f & 8));
+                const int nty = ty + dy, ntx = tx + dx;
+                if (need && nty >= 0 && nty < p.tiles_y && ntx >= 0 && ntx < p.tiles_x)
+                    p.stamp[nty * p.tiles_x + ntx] = next;
+            }
+        }
+    }
+}
+
+// Smallest valid elevation (as ordered bits of a float) and a NaN count,
+// for the monotonicity check.
+__global__ __launch_bounds__(256) void fill_range_kernel(FillParams p, unsigned *minbits, unsigned *nans)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
+    float v = INFINITY;
+    unsigned bad = 0;
+    if (c < p.cols && r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2) {
+        const float f = p.dem[(long long)r * p.pitch + c];
+        if (f != f) bad = 1;
+        else if (!is_nd(p, f)) v = f;
+    }
+    // order-preserving map of floats to unsigned
+    unsigned u = __float_as_uint(v);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(u, off);
+        u = o < u ? o : u;
+        bad += __shfl_xor(bad, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(minbits, u);
+        if (bad) atomicAdd(nans, bad);
+    }
+}
+
+}  // namespace
+
+}  // namespace pargo
+
+using pargo::fail;
+
+extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float *dem, double *filled_out,
+                                int *passes_out)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!g || !dem || !filled_out) return fail(ctx, PARGO_E_INVALID, "pargo_pit_remove: null argument");
+    if (g->rows < 1 || g->cols < 1) return fail(ctx, PARGO_E_INVALID, "empty grid (%d x %d)", g->rows, g->cols);
+    const long long pitch = g->pitch ? g->pitch : g->cols;
+    if (pitch < g->cols) return fail(ctx, PARGO_E_INVALID, "pitch %lld < cols %d", pitch, g->cols);
+    if (std::isnan(g->nodata)) return fail(ctx, PARGO_E_INVALID, "NaN NoData is not supported");
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    pargo::FillParams p{};
+    p.dem = dem;
+    p.w = filled_out;
+    p.pitch = pitch;
+    p.rows = g->rows;
+    p.cols = g->cols;
+    p.tiles_x = (g->cols + pargo::kTW - 1) / pargo::kTW;
+    p.tiles_y = (g->rows + pargo::kTH - 1) / pargo::kTH;
+    p.nodata = g->nodata;
+    // the reference keeps a cell when fabs(dem - noData) > 1e-6
+    // (pitRemoveOperator.cpp:40,53): NoData is the float interval with
+    // fabs <= 1e-6
+    pargo::nodata_interval(g->nodata, 0.000001, &p.nd_lo, &p.nd_hi, true);
+    if (p.nd_lo > p.nd_hi) {
+        p.nd_lo = INFINITY;
+        p.nd_hi = -INFINITY;
+    }
+    const int ntiles = p.tiles_x * p.tiles_y;
+    constexpr int kBatch = 8;      // passes between checks
+    constexpr int kMaxPasses = 1 << 20;
+    // scratch: [0] min bits, [1] NaN count, then per-pass counters, then stamps
+    const size_t counters = 2 + (size_t)kBatch;
+    rc = pargo::ensure_scratch(ctx, sizeof(unsigned) * counters + sizeof(int) * (size_t)ntiles + 256);
+    if (rc) return rc;
+    unsigned *scr = static_cast<unsigned *>(ctx->scratch);
+    unsigned *minbits = scr, *nans = scr + 1, *changed = scr + 2;
+    p.stamp = reinterpret_cast<int *>(scr + counters);
+    p.changed = changed;
+
+    // monotonicity check: every valid W lies in [min(dem, 10000), 10000]
+    unsigned host2[2] = {0xffffffffu, 0u};
+    PARGO_HIP(ctx, hipMemcpyAsync(scr, host2, sizeof(host2), hipMemcpyHostToDevice, ctx->stream));
+    dim3 rowsgrid((g->cols + 255) / 256, g->rows);
+    hipLaunchKernelGGL(pargo::fill_range_kernel, rowsgrid, dim3(256), 0, ctx->stream, p, minbits, nans);
+    PARGO_HIP(ctx, hipMemcpyAsync(host2, scr, sizeof(host2), hipMemcpyDeviceToHost, ctx->stream));
+    PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (host2[1]) return fail(ctx, PARGO_E_INVALID, "the DEM holds %u NaN cells: the reference never settles there", host2[1]);
+    {
+        unsigned u = host2[0];
+        u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+        float mn;
+        std::memcpy(&mn, &u, sizeof(mn));
+        // no valid cell leaves the smallest key's image at NaN
+        const bool any_valid = !std::isnan(mn);
+        const double lo = any_valid ? std::min((double)mn, 10000.0) : 0.0;
+        const bool ok = !any_valid || g->nodata < lo - 0.000001 || g->nodata > 10000.0 + 0.000001;
+        if (!ok)
+            return fail(ctx, PARGO_E_UNSUPPORTED,
+                        "NoData %.17g lies inside the range of filled values [%.17g, 10000]: the reference's "
+                        "result would depend on its sweep order",
+                        g->nodata, lo);
+    }
+    hipLaunchKernelGGL(pargo::fill_init_kernel, rowsgrid, dim3(256), 0, ctx->stream, p);
+    int pass = 0;
+    for (;;) {
+        // stamps start at 0 (every tile runs in pass 0); counters per batch
+        hipLaunchKernelGGL(pargo::fill_stamp_init_kernel, dim3(pass == 0 ? (std::max(ntiles, kBatch) + 255) / 256 : 1),
+                           dim3(256), 0, ctx->stream, p.stamp, pass == 0 ? ntiles : 0, changed, kBatch);
+        for (int k = 0; k < kBatch; k++)
+            hipLaunchKernelGGL(pargo::fill_pass_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, p, pass + k, pass);
+        unsigned h[kBatch];
+        PARGO_HIP(ctx, hipMemcpyAsync(h, changed, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        rc = pargo::check_launch(ctx, "pargo_pit_remove");
+        if (rc) return rc;
+        int settled = -1;
+        for (int k = 0; k < kBatch; k++)
+            if (h[k] == 0) {
+                settled = k;
+                break;
+            }
+        if (settled >= 0) {
+            pass += settled + 1;
+            break;
+        }
+        pass += kBatch;
+        if (pass >= kMaxPasses)
+            return fail(ctx, PARGO_E_NONTERM, "depression filling did not settle within %d passes", kMaxPasses);
+    }
+    if (passes_out) *passes_out = pass;
+    return PARGO_OK;
+}

@@ -51,8 +51,9 @@ int check_launch(pargo_ctx *ctx, const char *what);
 int ensure_scratch(pargo_ctx *ctx, size_t bytes);
 
 // Float interval [lo, hi] of the float32 values v for which the reference's
-// double test fabs((double)v - nodata) < eps holds (empty when lo > hi).
-void nodata_interval(double nodata, double eps, float *lo, float *hi);
+// double test fabs((double)v - nodata) < eps (<= eps when inclusive) holds
+// (empty when lo > hi).
+void nodata_interval(double nodata, double eps, float *lo, float *hi, bool inclusive = false);
 
 }  // namespace pargo
 

@@ -1,10 +1,13 @@
 """Python mirror of the hydrology operators over the C ABI: D8 contributing
-area (SCAOperator) and its input conversion.  Rasters are torch tensors on a
-HIP device; nothing here computes on the CPU.
+area (SCAOperator) with its input conversion, and depression filling
+(PitRemoveOperator).  Rasters are torch tensors on a HIP device; nothing here
+computes on the CPU.
 
 Reference interfaces mirrored:
   SCAOperator{d8Layer, scaLayer} + Run()
       apps/hydrology/scad8/scaOperator.h:13-41, scaOperator.cpp:5-131
+  PitRemoveOperator{demLayer, wdemLayer} + Run()
+      apps/hydrology/pitremove/pitRemoveOperator.h:87-113, pitRemoveOperator.cpp:3-72
 """
 import ctypes
 
@@ -77,3 +80,40 @@ class SCAOperator:
         flow = d8_classify(self._d8, self._nodata, ctx)
         self.sca = sca_d8(flow, self._nodata, None, self.mode, ctx)
         return self.sca
+
+
+def pit_remove(dem, nodata, ctx=None):
+    """Depression filling (PitRemoveOperator) of a float32 DEM on the device.
+    Returns (filled float64 with the reference's values, device passes)."""
+    import torch
+    _check(dem, torch.float32, "dem")
+    ctx = ctx or _lib.default_context(dem.device.index or 0)
+    ctx.bind_stream()
+    rows, cols = dem.shape
+    out = torch.empty((rows, dem.stride(0)), dtype=torch.float64, device=dem.device)[:, :cols]
+    g = grid_of(dem, 1.0, nodata)
+    passes = ctypes.c_int(0)
+    ctx.check(_lib.lib().pargo_pit_remove(ctx.h, ctypes.byref(g), _lib._ptr(dem), _lib._ptr(out),
+                                          ctypes.byref(passes)))
+    return out, passes.value
+
+
+class PitRemoveOperator:
+    """PitRemoveOperator: demLayer(layer) then Run(); the filled layer is
+    returned by run() and kept in .wdem."""
+
+    def __init__(self):
+        self._dem = None
+        self._nodata = None
+        self.wdem = None
+        self.passes = 0
+
+    def dem_layer(self, dem, nodata):
+        self._dem, self._nodata = dem, float(nodata)
+        return self
+
+    def run(self, ctx=None):
+        if self._dem is None:
+            raise RuntimeError("dem_layer() first")
+        self.wdem, self.passes = pit_remove(self._dem, self._nodata, ctx)
+        return self.wdem
