@@ -20,8 +20,8 @@
 // reference's while the order is free.
 //
 // Device order: 64 x 32 tiles.  A workgroup loads its tile and a one-cell
-// halo of W into LDS and repeats the update over the tile until nothing in
-// it changes; then it writes the tile back and, for each changed edge,
+// halo of W into LDS and sweeps the update over the tile (each wave down and
+// up its 8 rows) until nothing in it changes; then it writes the tile back and, for each changed edge,
 // schedules the neighbouring tiles for the next pass.  Passes repeat until
 // one changes nothing.  Tiles read each other's edges while they change;
 // that only feeds the monotone iteration earlier values, never wrong ones.
@@ -90,13 +90,14 @@ __global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, 
         const int r = r0 - 1 + lr, c = c0 - 1 + lc;
         sw[lr][lc] = (r >= 0 && r < p.rows && c >= 0 && c < p.cols) ? p.w[(long long)r * p.pitch + c] : p.nodata;
     }
-    // own cells: column tid % 64, rows tid / 64 + 4 k
+    // own cells: wave w owns tile rows 8w .. 8w+7, lane = column
     const int lc = tid % kTW + 1;
-    double d[kTH / 4];
-    unsigned live = 0;  // bit k: cell (rows tid/64 + 4k) is a valid interior cell
+    const int rbase = (tid / kTW) * 8;  // first tile row of this wave
+    double d[8];
+    unsigned live = 0;  // bit k: tile row rbase + k holds a valid interior cell
 #pragma unroll
-    for (int k = 0; k < kTH / 4; k++) {
-        const int r = r0 + tid / kTW + 4 * k, c = c0 + lc - 1;
+    for (int k = 0; k < 8; k++) {
+        const int r = r0 + rbase + k, c = c0 + lc - 1;
         d[k] = 0.0;
         if (r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2) {
             const float f = p.dem[(long long)r * p.pitch + c];
@@ -108,38 +109,52 @@ __global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, 
     }
     __syncthreads();
     unsigned touched = 0;  // bit k: the cell changed during this pass
+    // One update of the cell in tile row rbase + k; true if it changed.
+    auto update = [&](int k) -> bool {
+        if (!((live >> k) & 1u)) return false;
+        const int lr = rbase + k + 1;
+        const double w = sw[lr][lc];
+        if (w <= d[k]) return false;
+        bool to_dem = false;
+        double m = w;
+#pragma unroll
+        for (int n = 0; n < 9; n++) {
+            if (n == 4) continue;
+            const double wn = sw[lr + n / 3 - 1][lc + n % 3 - 1];
+            const double s = wn + kGap;
+            if (d[k] >= s || fabs(wn - p.nodata) < 0.000001) to_dem = true;
+            else if (m > s) m = s;
+        }
+        const double nw = to_dem ? d[k] : m;
+        if (nw == w) return false;
+        sw[lr][lc] = nw;
+        touched |= 1u << k;
+        return true;
+    };
+    // Each iteration sweeps the wave's 8 rows down and then up, one row per
+    // step: a lowered value travels 8 rows and up to 16 columns per
+    // iteration.  The wave-scope fences order the LDS writes of one step
+    // before the reads of the next (any order would still converge to the
+    // same values; the fences only make the sweeps effective).
     for (;;) {
         bool any = false;
 #pragma unroll
-        for (int k = 0; k < kTH / 4; k++) {
-            if (!((live >> k) & 1u)) continue;
-            const int lr = tid / kTW + 4 * k + 1;
-            const double w = sw[lr][lc];
-            if (w <= d[k]) continue;
-            bool to_dem = false;
-            double m = w;
+        for (int k = 0; k < 8; k++) {
+            any |= update(k);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
 #pragma unroll
-            for (int n = 0; n < 9; n++) {
-                if (n == 4) continue;
-                const double wn = sw[lr + n / 3 - 1][lc + n % 3 - 1];
-                const double s = wn + kGap;
-                if (d[k] >= s || fabs(wn - p.nodata) < 0.000001) to_dem = true;
-                else if (m > s) m = s;
-            }
-            const double nw = to_dem ? d[k] : m;
-            if (nw != w) {
-                sw[lr][lc] = nw;
-                any = true;
-                touched |= 1u << k;
-            }
+        for (int k = 7; k >= 0; k--) {
+            any |= update(k);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         if (!__syncthreads_or(any)) break;
     }
     unsigned flags = 0;
 #pragma unroll
-    for (int k = 0; k < kTH / 4; k++) {
+    for (int k = 0; k < 8; k++) {
         if (!((touched >> k) & 1u)) continue;
-        const int tr = tid / kTW + 4 * k, tc = lc - 1;
+        const int tr = rbase + k, tc = lc - 1;
         const int r = r0 + tr, c = c0 + tc;
         p.w[(long long)r * p.pitch + c] = sw[tr + 1][lc];
         flags |= (tr == 0 ? 1u : 0u) | (tr == kTH - 1 ? 2u : 0u) | (tc == 0 ? 4u : 0u) | (tc == kTW - 1 ? 8u : 0u) | 16u;
@@ -164,28 +179,43 @@ __global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, 
 }
 
 // Smallest valid elevation (as ordered bits of a float) and a NaN count,
-// for the monotonicity check.
+// for the monotonicity check: grid-stride over the interior, one atomic per
+// workgroup.
 __global__ __launch_bounds__(256) void fill_range_kernel(FillParams p, unsigned *minbits, unsigned *nans)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
-    float v = INFINITY;
-    unsigned bad = 0;
-    if (c < p.cols && r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2) {
+    __shared__ unsigned smin[4], sbad[4];
+    unsigned u = 0xffffffffu, bad = 0;
+    const long long n = (long long)p.rows * p.cols;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int r = (int)(i / p.cols), c = (int)(i - (long long)r * p.cols);
+        if (r < 1 || r > p.rows - 2 || c < 1 || c > p.cols - 2) continue;
         const float f = p.dem[(long long)r * p.pitch + c];
-        if (f != f) bad = 1;
-        else if (!is_nd(p, f)) v = f;
+        if (f != f) {
+            bad++;
+        } else if (!is_nd(p, f)) {
+            unsigned k = __float_as_uint(f);  // order-preserving map of floats to unsigned
+            k = (k & 0x80000000u) ? ~k : (k | 0x80000000u);
+            u = k < u ? k : u;
+        }
     }
-    // order-preserving map of floats to unsigned
-    unsigned u = __float_as_uint(v);
-    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned o = __shfl_xor(u, off);
         u = o < u ? o : u;
         bad += __shfl_xor(bad, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicMin(minbits, u);
-        if (bad) atomicAdd(nans, bad);
+        smin[threadIdx.x >> 6] = u;
+        sbad[threadIdx.x >> 6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = smin[0], b = sbad[0];
+        for (int k = 1; k < 4; k++) {
+            m = smin[k] < m ? smin[k] : m;
+            b += sbad[k];
+        }
+        atomicMin(minbits, m);
+        if (b) atomicAdd(nans, b);
     }
 }
 
@@ -239,7 +269,8 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
     unsigned host2[2] = {0xffffffffu, 0u};
     PARGO_HIP(ctx, hipMemcpyAsync(scr, host2, sizeof(host2), hipMemcpyHostToDevice, ctx->stream));
     dim3 rowsgrid((g->cols + 255) / 256, g->rows);
-    hipLaunchKernelGGL(pargo::fill_range_kernel, rowsgrid, dim3(256), 0, ctx->stream, p, minbits, nans);
+    const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
+    hipLaunchKernelGGL(pargo::fill_range_kernel, dim3(cus * 8), dim3(256), 0, ctx->stream, p, minbits, nans);
     PARGO_HIP(ctx, hipMemcpyAsync(host2, scr, sizeof(host2), hipMemcpyDeviceToHost, ctx->stream));
     PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (host2[1]) return fail(ctx, PARGO_E_INVALID, "the DEM holds %u NaN cells: the reference never settles there", host2[1]);
