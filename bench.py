@@ -14,6 +14,8 @@ slope sweep + D8 sweep, on a band of rows of the same DEM).  Unless
   adjoint  slope adjoint vs forward on an 8192^2 DEM (time ratio)
   idw      IDW of 20000 samples onto 4096^2 cells (K=12, p=2, 5 km)
   batch    512 uneven tiles (256..4096 cells a side) per-tile vs one graph
+  sca      D8 contributing area (scad8) of the headline DEM's dirType-8 codes
+  fill     depression filling (pitremove) of the headline DEM
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
@@ -155,6 +157,83 @@ def secondary_idw(torch, synth, IDWOperator, with_cpu):
     return res
 
 
+def secondary_sca(torch, synth, focal, hydro, z, with_cpu):
+    """D8 contributing area (scad8) from the dirType-8 codes of the
+    headline DEM, the reference's single sweep and the converging variant."""
+    rows, cols = z.shape
+    dem = torch.from_numpy(z).cuda()
+    code, mask = focal.d8(dem, CELL, NODATA, dirtype=8, want_mask=True)
+    del dem
+    res = {"config": "D8 contributing area of the 16384^2 headline DEM (dirType-8 codes + NoData mask in, "
+                     "float64 areas out)"}
+    cells = rows * cols
+    for mode in ("reference", "converge"):
+        ms = event_time(torch, lambda: hydro.sca_d8(code, NODATA, mask, mode), 5, warm=2)
+        res[mode + "_ms"] = round(ms, 3)
+        res[mode + "_Gcells_s"] = round(cells / ms / 1e6, 2)
+    # 1 B code + 1/8 B mask in, 8 B area out per cell
+    gbs = 9.125 * cells / res["reference_ms"] / 1e6
+    res["roofline"] = {"bound": "hbm (random 8-byte hops: one per chain link)", "achieved": round(gbs, 1),
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "algorithmic_bytes_per_cell": 9.125}
+    if with_cpu:
+        from oracle import pyoracle
+        import numpy as np
+        band = 768  # rows of the D8 layer as their own grid
+        d8 = focal.codes_to_reference(code[:band], focal.unpack_mask(mask[:band], cols), 8, NODATA).cpu().numpy()
+        t0 = time.perf_counter()
+        ref, _ = pyoracle.sca_d8(d8, NODATA, False)
+        dt = time.perf_counter() - t0
+        flow = hydro.d8_classify(torch.from_numpy(d8).cuda(), NODATA)
+        got = hydro.sca_d8(flow, NODATA).cpu().numpy()
+        res["cpu_baseline"] = {"value": round(band * cols / dt / 1e9, 5), "unit": "Gcells/s", "cores": 1,
+                               "kind": "port", "sample": "rows 0..%d of the D8 layer as a grid of its own, "
+                                                         "reference mode (the sweep order is sequential)" % (band - 1)}
+        res["parity_bitexact_on_sample"] = bool(np.array_equal(got, ref))
+    del code, mask
+    torch.cuda.empty_cache()
+    return res
+
+
+def secondary_fill(torch, synth, hydro, z, with_cpu):
+    """Depression filling (pitremove) of the headline DEM."""
+    rows, cols = z.shape
+    dem = torch.from_numpy(z).cuda()
+    hydro.pit_remove(dem, NODATA)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        w, passes = hydro.pit_remove(dem, NODATA)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = min(times) * 1e3
+    cells = rows * cols
+    filled = int((w > dem.double()).sum().item())
+    res = {"config": "pitremove of the 16384^2 headline DEM (float32 in, float64 out)", "ms": round(ms, 3),
+           "Gcells_s": round(cells / ms / 1e6, 2), "device_passes": passes, "cells_raised": filled}
+    gbs = 12.0 * cells / ms / 1e6  # 4 B DEM in, 8 B filled value out
+    res["roofline"] = {"bound": "hbm (repeated tile passes until nothing changes)", "achieved": round(gbs, 1),
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "algorithmic_bytes_per_cell": 12}
+    del w, dem
+    if with_cpu:
+        from oracle import pyoracle
+        import numpy as np
+        side = 384  # a window of the DEM as its own grid
+        win = np.ascontiguousarray(z[:side, :side])
+        t0 = time.perf_counter()
+        ref, sweeps, ok = pyoracle.pit_remove(win, NODATA)
+        dt = time.perf_counter() - t0
+        got, _ = hydro.pit_remove(torch.from_numpy(win).cuda(), NODATA)
+        res["cpu_baseline"] = {"value": round(side * side / dt / 1e9, 6), "unit": "Gcells/s", "cores": 1,
+                               "kind": "port", "sample": "the %dx%d top-left window as a grid of its own "
+                                                         "(%d Gauss-Seidel sweeps)" % (side, side, sweeps)}
+        res["parity_bitexact_on_sample"] = bool(ok and np.array_equal(got.cpu().numpy(), ref))
+    torch.cuda.empty_cache()
+    return res
+
+
 def secondary_batch(torch, batch):
     shapes = batch.tile_shapes(512, batch.synth.BASE_SEED + 1000)
     dems = [torch.from_numpy(batch.tile_dem(t, r, c)).cuda() for t, (r, c) in enumerate(shapes)]
@@ -196,7 +275,7 @@ def main():
 
     import numpy as np
     import torch
-    from pargo_amd import _lib, batch, focal, synth
+    from pargo_amd import _lib, batch, focal, hydro, synth
     from pargo_amd.idw import IDWOperator
 
     t0 = time.time()
@@ -316,6 +395,12 @@ def main():
         log("idw done (%.1fs)" % (time.time() - t0))
         sec["batch"] = secondary_batch(torch, batch)
         log("batch done (%.1fs)" % (time.time() - t0))
+        zh = synth.workload_dem("slope_d8")
+        sec["sca"] = secondary_sca(torch, synth, focal, hydro, zh, not a.no_cpu_baseline)
+        log("sca done (%.1fs)" % (time.time() - t0))
+        sec["fill"] = secondary_fill(torch, synth, hydro, zh, not a.no_cpu_baseline)
+        log("fill done (%.1fs)" % (time.time() - t0))
+        del zh
         out["secondary"] = sec
         out["adjoint_fwd_ratio"] = sec["adjoint"]["adjoint_fwd_ratio"]
     print(json.dumps(out), flush=True)
