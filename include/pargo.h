@@ -255,7 +255,33 @@ int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, 
                   const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
                   float *out, unsigned long long *candidates_out);
 
+/* Measured work of IDW per cell, the device form of the reference's
+ * preliminary experiment: with -dcmp space -writeLoad the reference stores
+ * each cell's CPU time in the output layer (apps/spatial/idw/
+ * idwOperator.cpp:439-491, pIDW.cpp:274-281); here work_out (device double,
+ * rows x cols) receives the number of samples the cell's search examined
+ * (0 for masked cells), a deterministic measure of the same cost.  out
+ * (may be NULL) receives the IDW values as pargo_idw_run would. */
+int pargo_idw_work(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                   const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
+                   float *out, double *work_out);
+
 /* ---- load-balanced decomposition ------------------------------------ */
+
+/* Intensity-ratio compute load of one data layer: Transformation::Operator
+ * (gpro/transformation.h:155-197) over the coarse grid of ComputeLayer::init
+ * (gpro/computeLayer.h:208-260).  The coarse grid has
+ * ceil(work_rows / g) x ceil(work_cols / g) cells; cell (i, j) sums, over
+ * the data cells of rows work_row0 + i*g .. work_row0 + (i+1)*g - 1 and the
+ * matching columns that lie inside the raster, valid_load for a value with
+ * fabs(v - grid->nodata) > 1e-7 and nodata_load otherwise, in row-major
+ * order.  data: device double raster (grid->pitch); load_out: device double
+ * array of load_rows x load_cols.  accumulate != 0 adds to load_out (the
+ * reference sums its data layers in order).  Loads must be >= 0.  With
+ * load_out NULL only *load_rows / *load_cols are set. */
+int pargo_intensity_load(pargo_ctx *ctx, const pargo_grid *grid, const double *data, int work_row0, int work_col0,
+                         int work_rows, int work_cols, int granularity, double nodata_load, double valid_load,
+                         int accumulate, double *load_out, int *load_rows, int *load_cols);
 
 /* Estimated compute load of IDW per cell of a coarse grid of
  * ceil(rows/granularity) x ceil(cols/granularity) cells, cell size

@@ -17,6 +17,9 @@
  *   - IDW load estimate, load-balanced bands ..... apps/spatial/idw/idwTransformation.cpp:3-62,
  *                                                 gpro/computeLayer.h:208-260,
  *                                                 gpro/deComposition.h:305-393 (valRowDcmp)
+ *   - intensity-ratio compute load ............... gpro/transformation.h:155-228
+ *   - D8 contributing area (scad8) ............... apps/hydrology/scad8/scaOperator.cpp:5-131
+ *   - depression filling (pitremove) ............. apps/hydrology/pitremove/pitRemoveOperator.cpp:3-72
  *   - bounds-checked row access .................. gpro/cellSpace.h:471-491
  *
  * Built with -O2 -ffp-contract=off and without -ffast-math so that every
@@ -28,7 +31,9 @@
  * reuses d[6], MD divides by the last neighbour's distance), brute-force KNN
  * for IDW with buffer 0 and a brute-force "blocks inside the buffer ring"
  * KNN for buffer > 0, and hand-derived tie-order cases for the strict-<
- * replacement rule.  The reference itself cannot be compiled in this
+ * replacement rule; scad8 against closed forms, brute-force path counts and
+ * two independent formulations; pitremove against a priority-flood solver;
+ * the intensity load against closed forms.  The reference itself cannot be compiled in this
  * container (it needs GDAL), so no function here is checked against a
  * reference binary.
  */
@@ -735,6 +740,43 @@ int oracle_val_row_dcmp(const double *load, int rows, int cols, double noData, i
     free(rowComptLoad);
     first[nSubSpcs] = rows;
     return nb == nSubSpcs ? 0 : -2;
+}
+
+/* ------------------------------------------------------------------ */
+/* Intensity-ratio compute load: Transformation::Operator and run()     */
+/* (gpro/transformation.h:155-228) over a ComputeLayer of granularity g */
+/* (gpro/computeLayer.h:208-260).                                       */
+/* ------------------------------------------------------------------ */
+#define CL_EPS 0.0000001 /* computeLayer.h:24 */
+
+/* data: nlayers raster layers of rows x cols doubles, back to back, each
+ * with its own NoData; the work rectangle starts at (row0, col0) and spans
+ * work_rows x work_cols; load: ceil(work_rows/g) x ceil(work_cols/g). */
+int oracle_intensity_load(const double *data, int nlayers, const double *nodata, int rows, int cols, int row0,
+                          int col0, int work_rows, int work_cols, int g, double nodataLoad, double validLoad,
+                          double *load)
+{
+    if (nodataLoad < 0 || validLoad < 0) return -1; /* transformation.h:168-171 */
+    const int lr = work_rows / g + (work_rows % g ? 1 : 0);
+    const int lc = work_cols / g + (work_cols % g ? 1 : 0);
+    const double grain = (double)g; /* _computGrain is a double */
+    for (int cRow = 0; cRow < lr; cRow++) {
+        for (int cCol = 0; cCol < lc; cCol++) {
+            double *out = load + (size_t)cRow * lc + cCol;
+            *out = 0.0;
+            for (int L = 0; L < nlayers; L++) {
+                const double *d = data + (size_t)L * rows * cols;
+                for (int dRow = (int)(cRow * grain + row0); dRow < (cRow + 1) * grain + row0; ++dRow) {
+                    for (int dCol = (int)(cCol * grain + col0); dCol < (cCol + 1) * grain + col0; ++dCol) {
+                        if (dRow > rows - 1 || dRow >= rows || dCol > cols - 1 || dCol >= cols) continue;
+                        if (fabs(d[(size_t)dRow * cols + dCol] - nodata[L]) > CL_EPS) *out += validLoad;
+                        else *out += nodataLoad;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

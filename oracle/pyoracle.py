@@ -39,6 +39,8 @@ def lib():
                                     ctypes.POINTER(ctypes.c_int)]
         L.oracle_pit_remove.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int)]
+        L.oracle_intensity_load.argtypes = [P, ctypes.c_int, P] + [ctypes.c_int] * 7 + [ctypes.c_double,
+                                                                                         ctypes.c_double, P]
         L.oracle_div_sqrt2.argtypes = [P, P, ctypes.c_int64]
         L.oracle_smpl1d_dcmp.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2
         L.oracle_num_threads.restype = ctypes.c_int
@@ -169,6 +171,21 @@ def pit_remove(dem, nodata=-9999.0, max_sweeps=0):
     if rc < 0:
         raise RuntimeError("oracle_pit_remove failed")
     return out, sw.value, rc == 0
+
+
+def intensity_load(layers, nodatas, row0, col0, work_rows, work_cols, g, nodata_load, valid_load):
+    """Intensity-ratio load of several equally sized float64 layers."""
+    a = np.ascontiguousarray(np.stack([np.asarray(l, dtype=np.float64) for l in layers]))
+    n, rows, cols = a.shape
+    nd = np.ascontiguousarray(nodatas, dtype=np.float64)
+    lr = work_rows // g + (1 if work_rows % g else 0)
+    lc = work_cols // g + (1 if work_cols % g else 0)
+    out = np.empty((lr, lc))
+    rc = lib().oracle_intensity_load(_p(a), n, _p(nd), rows, cols, row0, col0, work_rows, work_cols, g,
+                                     float(nodata_load), float(valid_load), _p(out))
+    if rc != 0:
+        raise ValueError("The load specified cannot be negative.")
+    return out
 
 
 def div_sqrt2(x):

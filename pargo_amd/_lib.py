@@ -58,6 +58,10 @@ _SIGS = {
                                  _P, _P]),
     "pargo_idw_load": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                   _P, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
+    "pargo_idw_work": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int, _c.c_int,
+                                  _P, _P]),
+    "pargo_intensity_load": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_double,
+                                        _c.c_double, _c.c_int, _P, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
     "pargo_load_row_bands": (_c.c_int, [_P, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _P]),
     "pargo_geotiff_info": (_c.c_int, [_c.c_char_p, _P]),
     "pargo_geotiff_read": (_c.c_int, [_c.c_char_p, _P, _P]),

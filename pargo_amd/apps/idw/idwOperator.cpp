@@ -185,18 +185,34 @@ bool IDWOperator::WorkDevice(const CoordBR &workBR, bool &ok)
                deviceOk(pargo_h2d(ctx, d_mask, mbuf.data(), mbuf.size() * sizeof(int)), "pargo_h2d(mask)");
     }
     std::vector<float> host(good ? n : 0);
-    good = good && deviceOk(pargo_idw_run(ctx, _plan, _nbrPoints, _idw_power, _idw_buffer, (const int32_t *)d_mask,
-                                          mrows, mcols, mnodata, (float *)d_out, (unsigned long long *)d_cand),
-                            "pargo_idw_run");
+    std::vector<double> work;
+    void *d_work = nullptr;
+    if (_writePreExpLoad) {
+        // The reference's preliminary experiment replaces each value by the
+        // cell's CPU time (idwOperator.cpp:455-458,482-484); the device
+        // records the samples each cell's search examined instead.
+        work.resize(good ? n : 0);
+        good = good && deviceOk(pargo_alloc(ctx, n * sizeof(double), &d_work), "pargo_alloc(work)") &&
+               deviceOk(pargo_idw_work(ctx, _plan, _nbrPoints, _idw_power, _idw_buffer, (const int32_t *)d_mask,
+                                       mrows, mcols, mnodata, (float *)d_out, (double *)d_work),
+                        "pargo_idw_work") &&
+               deviceOk(pargo_d2h(ctx, work.data(), d_work, n * sizeof(double)), "pargo_d2h(work)");
+    } else {
+        good = good && deviceOk(pargo_idw_run(ctx, _plan, _nbrPoints, _idw_power, _idw_buffer,
+                                              (const int32_t *)d_mask, mrows, mcols, mnodata, (float *)d_out,
+                                              (unsigned long long *)d_cand),
+                                "pargo_idw_run") &&
+               deviceOk(pargo_d2h(ctx, &_candidates, d_cand, sizeof(_candidates)), "pargo_d2h(count)");
+    }
     good = good && deviceOk(pargo_d2h(ctx, host.data(), d_out, n * sizeof(float)), "pargo_d2h(idw)") &&
-           deviceOk(pargo_d2h(ctx, &_candidates, d_cand, sizeof(_candidates)), "pargo_d2h(count)") &&
            deviceOk(pargo_sync(ctx), "pargo_sync");
-    for (void *p : {d_out, d_mask, d_cand})
+    for (void *p : {d_out, d_mask, d_cand, d_work})
         if (p) pargo_free(ctx, p);
     if (!good) return true;
     CellSpace<double> &idwL = *_pIDWLayer->cellSpace();
     for (int r = workBR.minIRow(); r <= workBR.maxIRow(); r++)
-        for (int c = workBR.minICol(); c <= workBR.maxICol(); c++) idwL[r][c] = (double)host[(size_t)r * cols + c];
+        for (int c = workBR.minICol(); c <= workBR.maxICol(); c++)
+            idwL[r][c] = _writePreExpLoad ? work[(size_t)r * cols + c] : (double)host[(size_t)r * cols + c];
     flag = true;
     ok = true;
     return true;

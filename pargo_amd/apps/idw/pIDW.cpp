@@ -6,8 +6,11 @@
 //                  with -writeLoad, and the rows are cut into bands of equal
 //                  load; with one process per GPU and one GPU the band is the
 //                  whole grid.
-// -writeLoad with -dcmp space asks the reference for per-cell CPU timings
-// instead of values; the GPU path has no per-cell clock and refuses it.
+// -writeLoad with -dcmp space is the reference's preliminary experiment: the
+// output file then holds each cell's cost instead of its value.  The
+// reference records CPU milliseconds; the device records the number of
+// samples each cell's search examined (its work, deterministic), which
+// -dcmp compute -readLoad balances the same way.
 #include <cstdlib>
 #include <cstring>
 
@@ -65,12 +68,6 @@ int main(int argc, char *argv[])
     if (!inputFileName || !outputFileName) Usage("-sample and -out are required");
     for (bool h : have)
         if (!h) Usage("-resolution, -fieldIndex, -idwExp, -searchPointNum, -searchRange and -blockSize are required");
-    if (decomposeBySpace && writeLoadPath) {
-        cerr << "[ERROR] -writeLoad with -dcmp space records per-cell CPU timings; the GPU path has no per-cell "
-                "clock. Use -dcmp compute to write the estimated load."
-             << endl;
-        return 1;
-    }
     if (!Application::START(HIP_Type, argc, argv)) return 1;
 
     double starttime;
@@ -92,6 +89,8 @@ int main(int argc, char *argv[])
             if (!maskLayer.readFile(maskFileName, ROWWISE_DCMP)) return 1;
             idwOper.maskLayer(maskLayer);
         }
+        // preliminary experiment: the output layer records each cell's cost
+        if (writeLoadPath) idwOper._writePreExpLoad = true;
         cout << "start computing" << endl;
         starttime = Application::wtime();
         if (!idwOper.Run()) return 1;

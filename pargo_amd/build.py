@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-ffp-contract=off"]
 
-HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip", "sca.hip", "fill.hip"]
+HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip", "sca.hip", "fill.hip", "compute.hip"]
 CXX_SOURCES = ["synth.cpp", "dcmp.cpp", "io.cpp", "../gpro/geotiff.cpp", "../gpro/shapefile.cpp"]
 
 

@@ -80,7 +80,8 @@ struct IdwParams {
     const double *__restrict__ v;
     const int *__restrict__ start;
     const int32_t *__restrict__ mask;
-    float *__restrict__ out;
+    float *__restrict__ out;             // may be null (work only)
+    double *__restrict__ work_out;       // per cell: samples examined, may be null
     unsigned long long *__restrict__ cand_out;
     const float *__restrict__ grad_out;  // adjoint: dL/d(out), rows x cols
     double *__restrict__ grad_acc;       // adjoint: per bucket slot
@@ -285,7 +286,8 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                 result = (float)acc;
             }
         }
-        if (!ADJOINT) p.out[(long long)row * p.cols + col] = result;
+        if (!ADJOINT && p.out) p.out[(long long)row * p.cols + col] = result;
+        if (!ADJOINT && p.work_out) p.work_out[(long long)row * p.cols + col] = (double)ncand;
     }
     if constexpr (ADJOINT) {
         __syncthreads();
@@ -572,6 +574,23 @@ int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, 
     rc = pargo::launch_idw<false>(p, ctx->stream);
     if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
     return pargo::check_launch(ctx, "pargo_idw_run");
+}
+
+int pargo_idw_work(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
+                   const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, float *out,
+                   double *work_out)
+{
+    pargo::IdwParams p;
+    int rc = pargo::idw_fill_params(ctx, plan, k, power, buffer, mask, mask_rows, mask_cols, mask_nodata, p);
+    if (rc) return rc;
+    if (!work_out) return fail(ctx, PARGO_E_INVALID, "pargo_idw_work: null work output");
+    rc = pargo::bind(ctx);
+    if (rc) return rc;
+    p.out = out;
+    p.work_out = work_out;
+    rc = pargo::launch_idw<false>(p, ctx->stream);
+    if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
+    return pargo::check_launch(ctx, "pargo_idw_work");
 }
 
 int pargo_idw_load(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int granularity, const int32_t *mask,

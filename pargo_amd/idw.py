@@ -91,6 +91,24 @@ class IDWOperator:
             return out, cand
         return out
 
+    def work(self, out=None):
+        """Measured work per cell: the number of samples each cell's search
+        examined (masked cells 0), float64 -- the device form of the
+        reference's per-cell timing layer (-dcmp space -writeLoad,
+        idwOperator.cpp:439-491).  out (float32, optional) gets the IDW
+        values of the same run."""
+        import torch
+        dev = torch.device("cuda", self.ctx.device)
+        work = torch.empty((self.rows, self.cols), dtype=torch.float64, device=dev)
+        mr = mc = 0
+        if self.mask is not None:
+            mr, mc = self.mask.shape
+        self.ctx.bind_stream()
+        self.ctx.check(_lib.lib().pargo_idw_work(self.ctx.h, self.plan, self.k, self.power, self.buffer,
+                                                 _lib._ptr(self.mask), mr, mc, self.mask_nodata, _lib._ptr(out),
+                                                 _lib._ptr(work)))
+        return work
+
     def compute_load(self, granularity=10, mask=None, mask_nodata=0.0, nodata=-9999.0):
         """Estimated load per granularity x granularity block of cells
         (IdwTransformation over a ComputeLayer, idwTransformation.cpp:3-62),

@@ -143,10 +143,21 @@ def test_idw_cli_compute_write_and_read_load(tmp_path):
     assert np.array_equal(io.read_geotiff(a)[0], io.read_geotiff(b)[0])
 
 
-def test_idw_cli_refuses_timing_load(tmp_path):
-    p = run([os.path.join(BIN, "idw"), "-sample", SAMPLES, "-out", str(tmp_path / "x.tif")] + IDW_ARGS +
-            ["-dcmp", "space", "-writeLoad", str(tmp_path / "t.tif")])
-    assert p.returncode != 0 and "per-cell" in p.stdout
+def test_idw_cli_preliminary_experiment(tmp_path):
+    """-dcmp space -writeLoad: the output holds each cell's cost (samples
+    examined) instead of its value; -dcmp compute -readLoad balances on it
+    and gives the normal IDW grid."""
+    work_f, out_f = str(tmp_path / "work.tif"), str(tmp_path / "x.tif")
+    run_ok([os.path.join(BIN, "idw"), "-sample", SAMPLES, "-out", work_f] + IDW_ARGS +
+           ["-dcmp", "space", "-writeLoad", str(tmp_path / "unused.tif")])
+    work, _ = io.read_geotiff(work_f)
+    xs, ys, vs = io.read_points(SAMPLES, 3)
+    _, cand, _ = po.idw(xs, ys, vs, 500.0, k=12, power=2, buffer=0.0, block_size=5000.0, stats=True)
+    assert np.array_equal(work.sum(axis=1).astype(np.int64), cand)
+    assert np.all(work == np.round(work)) and work.min() > 0
+    run_ok([os.path.join(BIN, "idw"), "-sample", SAMPLES, "-out", out_f] + IDW_ARGS +
+           ["-dcmp", "compute", "-readLoad", work_f])
+    check_idw(io.read_geotiff(out_f)[0], idw_reference(mask=False))
 
 
 @pytest.mark.parametrize("converge", [False, True])
