@@ -180,10 +180,17 @@ __device__ __forceinline__ unsigned push_bit(unsigned c, bool cond)
 // slots and rotates their roles statically, so nothing is copied per row.
 template <int CPT>
 struct Slot {
-    double z[CPT + 2];   // the row; replaced by z + 2*below once it is "up"
+    double z[CPT + 2];   // the row
+#if PARGO_D8_F32
+    float zf[CPT + 2];   // the row as loaded (float32)
+    float V[CPT];        // RN32(z[j+1] - below[j+1])
+    float SE[CPT + 1];   // RN32(RN32(z[i] - below[i+1]) * RN32(1/sqrt 2)), i = 0..CPT
+    float SW[CPT + 2];   // the same towards below[i-1], i = 1..CPT+1
+#else
     double V[CPT];       // RN(z[j+1] - below[j+1])
     double SE[CPT + 1];  // G(z[i] - below[i+1]), i = 0..CPT
     double SW[CPT + 2];  // G(z[i] - below[i-1]), i = 1..CPT+1
+#endif
     // per-position bits of this lane's CPT+2 values, filled only for rows
     // holding NoData or a special value: slope NoData (bits 0-7), D8 NoData
     // (8-15), |z| >= 2^40 or Inf (16-23)
@@ -226,6 +233,10 @@ __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CP
     }
 #pragma unroll
     for (int i = 0; i < CPT + 2; i++) o.z[i] = (double)f[i];
+#if PARGO_D8_F32
+#pragma unroll
+    for (int i = 0; i < CPT + 2; i++) o.zf[i] = f[i];
+#endif
 }
 
 // Per-cell slope for any method, in the reference's FP64 operation order
@@ -293,6 +304,27 @@ __device__ unsigned d8_exact(const double d[9])
     return code;
 }
 
+// D8 of one window on the reference's doubles (d8Operator.cpp:61-97) with
+// the correctly rounded division by sqrt(2.0) of div_sqrt2; window order
+// 0=NW 1=N 2=NE 3=W 4=C 5=E 6=SW 7=S 8=SE.  Returns the ESRI code (0 = flat).
+__device__ __forceinline__ unsigned d8_window_fp64(double d0, double d1, double d2, double d3, double d4, double d5,
+                                                   double d6, double d7, double d8)
+{
+    const double nw = div_sqrt2(d4 - d0), n = d4 - d1, ne = div_sqrt2(d4 - d2), w = d4 - d3;
+    const double e = d4 - d5, sw = div_sqrt2(d4 - d6), s_ = d4 - d7, se = div_sqrt2(d4 - d8);
+    const double M = vmax(vmax(vmax(vmax(nw, n), vmax(ne, w)), vmax(vmax(e, sw), vmax(s_, se))), 0.0);
+    unsigned c = 0;
+    c = push_bit(c, fabs(ne - M) < kD8Eps);  // NE  128
+    c = push_bit(c, fabs(n - M) < kD8Eps);   // N    64
+    c = push_bit(c, fabs(nw - M) < kD8Eps);  // NW   32
+    c = push_bit(c, fabs(w - M) < kD8Eps);   // W    16
+    c = push_bit(c, fabs(sw - M) < kD8Eps);  // SW    8
+    c = push_bit(c, fabs(s_ - M) < kD8Eps);  // S     4
+    c = push_bit(c, fabs(se - M) < kD8Eps);  // SE    2
+    c = push_bit(c, fabs(e - M) < kD8Eps);   // E     1
+    return (M < kD8Eps) ? 0u : c;
+}
+
 __device__ __forceinline__ float slope_unit(float s, int unit)
 {
     return unit == PARGO_UNIT_DEGREES ? atanf(s) * 57.295779513082323f : s;
@@ -341,6 +373,52 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
         }
     }
     // ---- D8 ---------------------------------------------------------------
+#if PARGO_D8_F32
+    // FP32 drops decide every cell whose FP64 answer they prove: with
+    // x64 the reference's drop and x32 ours, |x64 - x32| <= 2^-22 |x32|
+    // (+2^-149 near underflow) for cardinal and diagonal drops alike.  Let
+    // M = max(0, x32) and A = max |x32|.  M < 0.9999e-6 proves the
+    // reference's max < 1e-6 (flat).  M >= 1.0001e-6 together with
+    // RN32(M - x) >= RN32(2^-20 A + 1.0001e-6) for every drop but one proves
+    // that drop is the reference's unique max and no other drop is within
+    // 1e-6 of it.  Any other cell is ambiguous and recomputed in FP64.
+    unsigned amb = 0;  // bit j: cell j needs the FP64 window
+    if (DO_D8) {
+        float H[CPT + 1];
+#pragma unroll
+        for (int i = 0; i <= CPT; i++) H[i] = mid.zf[i] - mid.zf[i + 1];
+#pragma unroll
+        for (int j = 0; j < CPT; j++) mid.V[j] = mid.zf[j + 1] - dn.zf[j + 1];
+#pragma unroll
+        for (int i = 0; i <= CPT; i++) mid.SE[i] = (mid.zf[i] - dn.zf[i + 1]) * kInvSqrt2f;
+#pragma unroll
+        for (int i = 1; i <= CPT + 1; i++) mid.SW[i] = (mid.zf[i] - dn.zf[i - 1]) * kInvSqrt2f;
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int i = j + 1;
+            const float e = H[i], w = -H[i - 1], s_ = mid.V[j], n = -up.V[j];
+            const float se = mid.SE[i], nw = -up.SE[i - 1], sw = mid.SW[i], ne = -up.SW[i + 1];
+            const float M = fmaxf(fmaxf(fmaxf(fmaxf(e, w), fmaxf(s_, n)), fmaxf(fmaxf(se, nw), fmaxf(sw, ne))), 0.0f);
+            const float A = fmaxf(fmaxf(fmaxf(fabsf(e), fabsf(w)), fmaxf(fabsf(s_), fabsf(n))),
+                                  fmaxf(fmaxf(fabsf(se), fabsf(nw)), fmaxf(fabsf(sw), fabsf(ne))));
+            const float T = __builtin_fmaf(0x1p-20f, A, kD8SureF);
+            // candidates for the max, in the order that leaves the ESRI code
+            unsigned c = 0;
+            c = push_bit(c, !(M - ne >= T));   // NE  128
+            c = push_bit(c, !(M - n >= T));    // N    64
+            c = push_bit(c, !(M - nw >= T));   // NW   32
+            c = push_bit(c, !(M - w >= T));    // W    16
+            c = push_bit(c, !(M - sw >= T));   // SW    8
+            c = push_bit(c, !(M - s_ >= T));   // S     4
+            c = push_bit(c, !(M - se >= T));   // SE    2
+            c = push_bit(c, !(M - e >= T));    // E     1
+            const bool flat = M < kD8FlatF;
+            const bool sure = (M >= kD8SureF) && __builtin_popcount(c) == 1;
+            cd[j] = flat ? 0u : c;
+            amb |= (unsigned)(!flat && !sure) << j;
+        }
+    }
+#else
     if (DO_D8) {
         double H[CPT + 1];
 #pragma unroll
@@ -372,6 +450,7 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
             cd[j] = (M < kD8Eps) ? 0u : c;
         }
     }
+#endif
 
     // ---- NoData, border and special windows (rare) -------------------------
     // (windows with a special value keep their fast-path values until the
@@ -390,6 +469,20 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
             special |= ((flw >> (j + 16)) & 7u) && !((!DO_SLOPE || nds) && (!DO_D8 || n8));
         }
     }
+#if PARGO_D8_F32
+    if (DO_D8) {
+        amb &= ~(ndflags >> 8);  // NoData windows take NoData whatever the code
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(amb != 0) != 0, 0)) {
+#pragma unroll
+            for (int j = 0; j < CPT; j++) {
+                const int i = j + 1;
+                const unsigned c64 = d8_window_fp64(up.z[i - 1], up.z[i], up.z[i + 1], mid.z[i - 1], mid.z[i],
+                                                    mid.z[i + 1], dn.z[i - 1], dn.z[i], dn.z[i + 1]);
+                if ((amb >> j) & 1u) cd[j] = c64;
+            }
+        }
+    }
+#endif
     if (DO_SLOPE && p.unit == PARGO_UNIT_DEGREES) {
 #pragma unroll
         for (int j = 0; j < CPT; j++) sl[j] = slope_unit(sl[j], p.unit);
@@ -459,12 +552,21 @@ template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
 __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
 {
     if (DO_D8) {
+#if PARGO_D8_F32
+#pragma unroll
+        for (int j = 0; j < CPT; j++) up.V[j] = up.zf[j + 1] - below.zf[j + 1];
+#pragma unroll
+        for (int i = 0; i <= CPT; i++) up.SE[i] = (up.zf[i] - below.zf[i + 1]) * kInvSqrt2f;
+#pragma unroll
+        for (int i = 1; i <= CPT + 1; i++) up.SW[i] = (up.zf[i] - below.zf[i - 1]) * kInvSqrt2f;
+#else
 #pragma unroll
         for (int j = 0; j < CPT; j++) up.V[j] = up.z[j + 1] - below.z[j + 1];
 #pragma unroll
         for (int i = 0; i <= CPT; i++) up.SE[i] = div_sqrt2(up.z[i] - below.z[i + 1]);
 #pragma unroll
         for (int i = 1; i <= CPT + 1; i++) up.SW[i] = div_sqrt2(up.z[i] - below.z[i - 1]);
+#endif
     }
 }
 

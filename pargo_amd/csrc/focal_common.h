@@ -14,6 +14,15 @@ constexpr double kInvSqrt2 = 0x1.6a09e667f3bccp-1;
 // The D8 epsilon 0.000001 (apps/hydrology/flowdird8/d8Operator.cpp:3).
 constexpr double kD8Eps = 0.000001;
 
+// FP32 D8 filter (focal.hip): RN32(1/sqrt 2) and the thresholds that keep
+// the filter's decisions provably those of the FP64 tests.
+#ifndef PARGO_D8_F32
+#define PARGO_D8_F32 1
+#endif
+constexpr float kInvSqrt2f = 0x1.6a09e6p-1f;
+constexpr float kD8FlatF = 0.9999e-6f;
+constexpr float kD8SureF = 1.0001e-6f;
+
 // RN(x / kSqrt2) for finite x derived from float32 data (|x| <= 2^129,
 // x = 0 or |x| >= 2^-149, so nothing below underflows).
 // q0 = RN(x*y) with y = RN(1/s) is within 0.69 ulp of x/s because
