@@ -377,11 +377,13 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
     // FP32 drops decide every cell whose FP64 answer they prove: with
     // x64 the reference's drop and x32 ours, |x64 - x32| <= 2^-22 |x32|
     // (+2^-149 near underflow) for cardinal and diagonal drops alike.  Let
-    // M = max(0, x32) and A = max |x32|.  M < 0.9999e-6 proves the
-    // reference's max < 1e-6 (flat).  M >= 1.0001e-6 together with
-    // RN32(M - x) >= RN32(2^-20 A + 1.0001e-6) for every drop but one proves
-    // that drop is the reference's unique max and no other drop is within
-    // 1e-6 of it.  Any other cell is ambiguous and recomputed in FP64.
+    // M = max(0, x32), A = max |x32| and L = RN32(M - RN32(2^-19 A +
+    // 1.0001e-6)).  M < 0.9999e-6 proves the reference's max < 1e-6 (flat).
+    // x <= L gives M64 - x64 >= 1.0001e-6 (1 - 2^-23) + A (2^-19 (1 - 2^-23)
+    // - 2^-21 - 2^-24) > 1e-6, so M >= 1.0001e-6 with x <= L for every drop
+    // but one proves that drop is the reference's unique max and no other
+    // drop is within 1e-6 of it.  Any other cell is ambiguous and
+    // recomputed in FP64.
     unsigned amb = 0;  // bit j: cell j needs the FP64 window
     if (DO_D8) {
         float H[CPT + 1];
@@ -401,17 +403,17 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
             const float M = fmaxf(fmaxf(fmaxf(fmaxf(e, w), fmaxf(s_, n)), fmaxf(fmaxf(se, nw), fmaxf(sw, ne))), 0.0f);
             const float A = fmaxf(fmaxf(fmaxf(fabsf(e), fabsf(w)), fmaxf(fabsf(s_), fabsf(n))),
                                   fmaxf(fmaxf(fabsf(se), fabsf(nw)), fmaxf(fabsf(sw), fabsf(ne))));
-            const float T = __builtin_fmaf(0x1p-20f, A, kD8SureF);
+            const float L = M - __builtin_fmaf(0x1p-19f, A, kD8SureF);
             // candidates for the max, in the order that leaves the ESRI code
             unsigned c = 0;
-            c = push_bit(c, !(M - ne >= T));   // NE  128
-            c = push_bit(c, !(M - n >= T));    // N    64
-            c = push_bit(c, !(M - nw >= T));   // NW   32
-            c = push_bit(c, !(M - w >= T));    // W    16
-            c = push_bit(c, !(M - sw >= T));   // SW    8
-            c = push_bit(c, !(M - s_ >= T));   // S     4
-            c = push_bit(c, !(M - se >= T));   // SE    2
-            c = push_bit(c, !(M - e >= T));    // E     1
+            c = push_bit(c, !(ne <= L));   // NE  128
+            c = push_bit(c, !(n <= L));    // N    64
+            c = push_bit(c, !(nw <= L));   // NW   32
+            c = push_bit(c, !(w <= L));    // W    16
+            c = push_bit(c, !(sw <= L));   // SW    8
+            c = push_bit(c, !(s_ <= L));   // S     4
+            c = push_bit(c, !(se <= L));   // SE    2
+            c = push_bit(c, !(e <= L));    // E     1
             const bool flat = M < kD8FlatF;
             const bool sure = (M >= kD8SureF) && __builtin_popcount(c) == 1;
             cd[j] = flat ? 0u : c;
