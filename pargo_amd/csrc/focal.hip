@@ -626,8 +626,11 @@ __device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, i
 }
 
 // One raster: block = 4 waves side by side, one row strip per block.
+// Four waves per SIMD (at most 128 VGPRs, a few spilled to scratch) ran
+// faster than three without spills: the kernel issues VALU work for most
+// of its time and a fourth wave fills the dependency stalls.
 template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
-__global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void focal_kernel(FocalParams p)
 {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
