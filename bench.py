@@ -342,7 +342,7 @@ def main():
                    "io": "f32 DEM in, f32 slope + u8 D8 code out; D8 decisions in f64"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "kernel": "focal_kernel<4,true,true,true,true> + focal_fixup_kernel (empty queue)",
+                     "kernel": "focal_kernel_fused<4,true> + focal_fixup_kernel (empty queue)",
                      "step_ms": round(avg_ms, 5), "algorithmic_bytes_per_cell": BYTES_PER_CELL},
     }
 

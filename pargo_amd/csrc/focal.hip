@@ -626,11 +626,8 @@ __device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, i
 }
 
 // One raster: block = 4 waves side by side, one row strip per block.
-// Four waves per SIMD (at most 128 VGPRs, a few spilled to scratch) ran
-// faster than three without spills: the kernel issues VALU work for most
-// of its time and a fourth wave fills the dependency stalls.
 template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void focal_kernel(FocalParams p)
+__device__ __forceinline__ void focal_raster(const FocalParams &p)
 {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -638,6 +635,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int rbeg = blockIdx.y * p.rows_per_strip;
     const int rend = min(rbeg + p.rows_per_strip, p.rows);
     focal_strip<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
+}
+
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+__global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
+{
+    focal_raster<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p);
+}
+
+// The fused Horn slope + D8 pass at four waves per SIMD (at most 128 VGPRs,
+// a few spilled to scratch): it ran faster than at three waves without
+// spills, since the pass issues VALU work most of the time and a fourth
+// wave fills the dependency stalls.  The other variants keep three.
+template <int CPT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void focal_kernel_fused(FocalParams p)
+{
+    focal_raster<CPT, VEC, true, true, true>(p);
 }
 
 // Many rasters in one persistent launch: every wave takes (tile, column
@@ -751,8 +764,13 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
     const int cols_per_block = threads * cfg.cpt;
     dim3 grid((p.cols + cols_per_block - 1) / cols_per_block, (p.rows + cfg.rows_per_strip - 1) / cfg.rows_per_strip);
     if (cfg.cpt == 4) {
-        if (vec) hipLaunchKernelGGL((focal_kernel<4, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((focal_kernel<4, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
+        if constexpr (DO_SLOPE && DO_D8 && HORN) {
+            if (vec) hipLaunchKernelGGL((focal_kernel_fused<4, true>), grid, block, 0, s, p);
+            else hipLaunchKernelGGL((focal_kernel_fused<4, false>), grid, block, 0, s, p);
+        } else {
+            if (vec) hipLaunchKernelGGL((focal_kernel<4, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
+            else hipLaunchKernelGGL((focal_kernel<4, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
+        }
     } else {
         if (vec) hipLaunchKernelGGL((focal_kernel<2, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((focal_kernel<2, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
