@@ -259,7 +259,7 @@ def main():
     # default warm-up covers them
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--size", type=int, default=16384)
-    ap.add_argument("--cpu-rows", type=int, default=4096, help="rows of the DEM in the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=16384, help="rows of the DEM in the CPU baseline sample")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
