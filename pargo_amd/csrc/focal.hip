@@ -17,13 +17,16 @@
 // from cross-lane shuffles; only lanes 0 and 63 read their outer halo
 // column from memory.  Rows of a wave are coalesced 64*CPT*4-byte segments.
 //
-// D8 decisions are made on exactly the doubles the reference compares:
-// drops RN(z4 - zi) for cardinal neighbours and RN(RN(z4 - zi) / sqrt(2.0))
-// for diagonal ones, max over them and 0, then the 1e-6 tests.  A drop
-// shared by two neighbouring cells is computed once (RN(a-b) = -RN(b-a),
-// and the division routine is odd), and the division by sqrt(2.0) is the
-// correctly rounded Markstein sequence of focal_common.h.  Windows that hold
-// NaN, Inf or |z| >= 2^40 take a per-cell path with IEEE division instead.
+// D8 codes are those of the doubles the reference compares: drops
+// RN(z4 - zi) for cardinal neighbours and RN(RN(z4 - zi) / sqrt(2.0)) for
+// diagonal ones, max over them and 0, then the 1e-6 tests.  An FP32 filter
+// on the same drops settles every cell whose code it can prove (almost
+// all); a lane holding any other cell recomputes its cells on the doubles.
+// A drop shared by two neighbouring cells is computed once (RN(a-b) =
+// -RN(b-a), and the division routine is odd), and the division by
+// sqrt(2.0) is the correctly rounded Markstein sequence of focal_common.h.
+// Windows that hold Inf or |z| >= 2^40 take a per-cell path with IEEE
+// division instead.
 //
 // The Horn numerators are formed in FP64 (exact for float32 input of
 // moderate range), the square root in FP32: relative error below 5e-7.
@@ -166,6 +169,45 @@ __device__ __forceinline__ double vmax_neg_zero(double a, double b)
     return r;
 }
 
+// FP32 three-way max/min as single instructions, again without the quieting
+// LLVM adds in front of fmaxf/fminf.  IEEE mode: a quiet NaN operand is
+// skipped and a signalling one makes the result a quiet NaN; every caller
+// either feeds results of FP32 arithmetic (quiet) or treats a NaN result as
+// "take the careful path".
+__device__ __forceinline__ float max3f(float a, float b, float c)
+{
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float min3f(float a, float b, float c)
+{
+    float r;
+    asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+// max(-a, -b, -c)
+__device__ __forceinline__ float max3f_neg(float a, float b, float c)
+{
+    float r;
+    asm("v_max3_f32 %0, -%1, -%2, -%3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+// max(a, -b, 0)
+__device__ __forceinline__ float max3f_pos_neg_zero(float a, float b)
+{
+    float r;
+    asm("v_max3_f32 %0, %1, -%2, 0" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// max(a, -b)
+__device__ __forceinline__ float maxf_neg(float a, float b)
+{
+    float r;
+    asm("v_max_f32 %0, %1, -%2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // c = 2c + (lane bit of `mask`): one v_addc with the lane mask as carry-in.
 __device__ __forceinline__ unsigned push_bit(unsigned c, bool cond)
 {
@@ -181,27 +223,23 @@ __device__ __forceinline__ unsigned push_bit(unsigned c, bool cond)
 template <int CPT>
 struct Slot {
     double z[CPT + 2];   // the row
-#if PARGO_D8_F32
     float zf[CPT + 2];   // the row as loaded (float32)
     float V[CPT];        // RN32(z[j+1] - below[j+1])
     float SE[CPT + 1];   // RN32(RN32(z[i] - below[i+1]) * RN32(1/sqrt 2)), i = 0..CPT
     float SW[CPT + 2];   // the same towards below[i-1], i = 1..CPT+1
-#else
-    double V[CPT];       // RN(z[j+1] - below[j+1])
-    double SE[CPT + 1];  // G(z[i] - below[i+1]), i = 0..CPT
-    double SW[CPT + 2];  // G(z[i] - below[i-1]), i = 1..CPT+1
-#endif
     // per-position bits of this lane's CPT+2 values, filled only for rows
     // holding NoData or a special value: slope NoData (bits 0-7), D8 NoData
     // (8-15), |z| >= 2^40 or Inf (16-23)
     unsigned flags;
 };
 
-// Converts a loaded row.  The common case costs a min, a max and a max of
-// magnitudes over the CPT+2 values; only rows holding NoData or values with
-// |z| >= 2^40 (incl. Inf) get per-position bits.  NaN needs no flag: the
-// FP64 max and tie tests skip NaN drops exactly as the reference's > and <
-// comparisons do, and NaN is never NoData.
+// Converts a loaded row.  The common case costs a min and a max over the
+// CPT+2 values (three-way instructions) and max(mx, -mn) = max |z|; only
+// rows holding NoData or values with |z| >= 2^40 (incl. Inf) get
+// per-position bits.  A NaN that the min/max skip needs no flag: the FP32
+// filter and the FP64 tests skip NaN drops exactly as the reference's > and
+// < comparisons do, and NaN is never NoData.  A signalling NaN turns mn/mx
+// into NaN, which only sends the row to the per-position test.
 template <int CPT>
 __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CPT> &raw, int lane, Slot<CPT> &o)
 {
@@ -212,13 +250,14 @@ __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CP
     f[CPT + 1] = (lane == 63) ? raw.hl : right;
 #pragma unroll
     for (int k = 0; k < CPT; k++) f[k + 1] = raw.v[k];
-    float mn = f[0], mx = f[0], am = __builtin_fabsf(f[0]);
+    float mn = f[0], mx = f[0];
 #pragma unroll
-    for (int i = 1; i < CPT + 2; i++) {
-        mn = fminf(mn, f[i]);
-        mx = fmaxf(mx, f[i]);
-        am = fmaxf(am, __builtin_fabsf(f[i]));
+    for (int i = 1; i < CPT + 2; i += 2) {
+        const float b = f[i], c = (i + 1 < CPT + 2) ? f[i + 1] : f[i];
+        mn = min3f(mn, b, c);
+        mx = max3f(mx, b, c);
     }
+    const float am = maxf_neg(mx, mn);
     const bool exc = !(mx < p.ndu_lo || mn > p.ndu_hi) || !(am < 0x1p40f);
     o.flags = 0;
     if (exc) {
@@ -233,10 +272,8 @@ __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CP
     }
 #pragma unroll
     for (int i = 0; i < CPT + 2; i++) o.z[i] = (double)f[i];
-#if PARGO_D8_F32
 #pragma unroll
     for (int i = 0; i < CPT + 2; i++) o.zf[i] = f[i];
-#endif
 }
 
 // Per-cell slope for any method, in the reference's FP64 operation order
@@ -373,18 +410,28 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
         }
     }
     // ---- D8 ---------------------------------------------------------------
-#if PARGO_D8_F32
-    // FP32 drops decide every cell whose FP64 answer they prove: with
-    // x64 the reference's drop and x32 ours, |x64 - x32| <= 2^-22 |x32|
-    // (+2^-149 near underflow) for cardinal and diagonal drops alike.  Let
-    // M = max(0, x32), A = max |x32| and L = RN32(M - RN32(2^-19 A +
-    // 1.0001e-6)).  M < 0.9999e-6 proves the reference's max < 1e-6 (flat).
-    // x <= L gives M64 - x64 >= 1.0001e-6 (1 - 2^-23) + A (2^-19 (1 - 2^-23)
-    // - 2^-21 - 2^-24) > 1e-6, so M >= 1.0001e-6 with x <= L for every drop
-    // but one proves that drop is the reference's unique max and no other
-    // drop is within 1e-6 of it.  Any other cell is ambiguous and
-    // recomputed in FP64.
-    unsigned amb = 0;  // bit j: cell j needs the FP64 window
+    // FP32 drops decide every cell whose FP64 answer they prove.  With x64
+    // the reference's drop and x32 ours, |x64 - x32| <= 2^-22 |x32|
+    // (+2^-149 near underflow) for cardinal and diagonal drops alike, and
+    // x32, x64 have the same sign.  Let M = max(0, x32) over the 8 drops,
+    // G = the float just below 0.9999e-6, K = RN32(1.0001e-6) and
+    // L = max(RN32(M (1 - 2^-20) - K), G); a drop is a candidate when
+    // x32 > L (a NaN drop never is: the reference's > and < skip it too).
+    //  * M <= G proves the reference's max below 1e-6 (flat, code 0), and
+    //    then no drop exceeds L: the candidate set is empty.
+    //  * M >= 2.00025e-6 gives L = RN32(M (1 - 2^-20) - K) >= G and M > K.
+    //    A negative non-candidate has x64 <= 0, so M64 - x64 >= M64 > 1e-6;
+    //    a non-negative one has x32 <= L, and with the rounding of L
+    //    M64 - x64 >= M (1.75 2^-22) + K > 1e-6.  So if exactly one drop is
+    //    a candidate it is the reference's unique max, and its code bit is
+    //    the reference's code.
+    //  * M in between (within 0.50025e-6 of 1.5e-6), or two or more
+    //    candidates: the lane recomputes its cells in FP64.
+    // Every FP64 code bit is a candidate, so the candidate set is the code
+    // itself whenever it has one bit.
+    bool amb = false;  // this lane recomputes its cells with the FP64 windows
+    unsigned popmax = 0;
+    float band = 0.0f;
     if (DO_D8) {
         float H[CPT + 1];
 #pragma unroll
@@ -395,64 +442,30 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
         for (int i = 0; i <= CPT; i++) mid.SE[i] = (mid.zf[i] - dn.zf[i + 1]) * kInvSqrt2f;
 #pragma unroll
         for (int i = 1; i <= CPT + 1; i++) mid.SW[i] = (mid.zf[i] - dn.zf[i - 1]) * kInvSqrt2f;
-#pragma unroll
-        for (int j = 0; j < CPT; j++) {
-            const int i = j + 1;
-            const float e = H[i], w = -H[i - 1], s_ = mid.V[j], n = -up.V[j];
-            const float se = mid.SE[i], nw = -up.SE[i - 1], sw = mid.SW[i], ne = -up.SW[i + 1];
-            const float M = fmaxf(fmaxf(fmaxf(fmaxf(e, w), fmaxf(s_, n)), fmaxf(fmaxf(se, nw), fmaxf(sw, ne))), 0.0f);
-            const float A = fmaxf(fmaxf(fmaxf(fabsf(e), fabsf(w)), fmaxf(fabsf(s_), fabsf(n))),
-                                  fmaxf(fmaxf(fabsf(se), fabsf(nw)), fmaxf(fabsf(sw), fabsf(ne))));
-            const float L = M - __builtin_fmaf(0x1p-19f, A, kD8SureF);
-            // candidates for the max, in the order that leaves the ESRI code
-            unsigned c = 0;
-            c = push_bit(c, !(ne <= L));   // NE  128
-            c = push_bit(c, !(n <= L));    // N    64
-            c = push_bit(c, !(nw <= L));   // NW   32
-            c = push_bit(c, !(w <= L));    // W    16
-            c = push_bit(c, !(sw <= L));   // SW    8
-            c = push_bit(c, !(s_ <= L));   // S     4
-            c = push_bit(c, !(se <= L));   // SE    2
-            c = push_bit(c, !(e <= L));    // E     1
-            const bool flat = M < kD8FlatF;
-            const bool sure = (M >= kD8SureF) && __builtin_popcount(c) == 1;
-            cd[j] = flat ? 0u : c;
-            amb |= (unsigned)(!flat && !sure) << j;
-        }
-    }
-#else
-    if (DO_D8) {
-        double H[CPT + 1];
-#pragma unroll
-        for (int i = 0; i <= CPT; i++) H[i] = mid.z[i] - mid.z[i + 1];
-#pragma unroll
-        for (int j = 0; j < CPT; j++) mid.V[j] = mid.z[j + 1] - dn.z[j + 1];
-#pragma unroll
-        for (int i = 0; i <= CPT; i++) mid.SE[i] = div_sqrt2(mid.z[i] - dn.z[i + 1]);
-#pragma unroll
-        for (int i = 1; i <= CPT + 1; i++) mid.SW[i] = div_sqrt2(mid.z[i] - dn.z[i - 1]);
+        band = __builtin_huge_valf();
 #pragma unroll
         for (int j = 0; j < CPT; j++) {
             const int i = j + 1;
             // drops  E=H[i]  W=-H[i-1]  S=mid.V[j]  N=-up.V[j]
             //        SE=mid.SE[i]  NW=-up.SE[i-1]  SW=mid.SW[i]  NE=-up.SW[i+1]
-            // max(0, drops) = max(0, max(E,S,SE,SW), -min(-W,-N,-NW,-NE))
-            const double a = vmax(vmax(H[i], mid.V[j]), vmax(mid.SE[i], mid.SW[i]));
-            const double b = vmin(vmin(H[i - 1], up.V[j]), vmin(up.SE[i - 1], up.SW[i + 1]));
-            const double M = vmax_neg_zero(a, b);
+            const float M = max3f(max3f(H[i], mid.V[j], mid.SE[i]), max3f_neg(H[i - 1], up.V[j], up.SE[i - 1]),
+                                  max3f_pos_neg_zero(mid.SW[i], up.SW[i + 1]));
+            const float L = fmaxf(__builtin_fmaf(M, kD8Shrink, -kD8SureF), kD8FloorF);
+            // candidates, in the order that leaves the ESRI code
             unsigned c = 0;
-            c = push_bit(c, fabs(-up.SW[i + 1] - M) < kD8Eps);  // NE  128
-            c = push_bit(c, fabs(-up.V[j] - M) < kD8Eps);       // N    64
-            c = push_bit(c, fabs(-up.SE[i - 1] - M) < kD8Eps);  // NW   32
-            c = push_bit(c, fabs(-H[i - 1] - M) < kD8Eps);      // W    16
-            c = push_bit(c, fabs(mid.SW[i] - M) < kD8Eps);      // SW    8
-            c = push_bit(c, fabs(mid.V[j] - M) < kD8Eps);       // S     4
-            c = push_bit(c, fabs(mid.SE[i] - M) < kD8Eps);      // SE    2
-            c = push_bit(c, fabs(H[i] - M) < kD8Eps);           // E     1
-            cd[j] = (M < kD8Eps) ? 0u : c;
+            c = push_bit(c, -up.SW[i + 1] > L);  // NE  128
+            c = push_bit(c, -up.V[j] > L);       // N    64
+            c = push_bit(c, -up.SE[i - 1] > L);  // NW   32
+            c = push_bit(c, -H[i - 1] > L);      // W    16
+            c = push_bit(c, mid.SW[i] > L);      // SW    8
+            c = push_bit(c, mid.V[j] > L);       // S     4
+            c = push_bit(c, mid.SE[i] > L);      // SE    2
+            c = push_bit(c, H[i] > L);           // E     1
+            cd[j] = c;
+            popmax = max(popmax, (unsigned)__builtin_popcount(c));
+            band = fminf(band, __builtin_fabsf(M - kD8BandMid));
         }
     }
-#endif
 
     // ---- NoData, border and special windows (rare) -------------------------
     // (windows with a special value keep their fast-path values until the
@@ -471,20 +484,25 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
             special |= ((flw >> (j + 16)) & 7u) && !((!DO_SLOPE || nds) && (!DO_D8 || n8));
         }
     }
-#if PARGO_D8_F32
     if (DO_D8) {
-        amb &= ~(ndflags >> 8);  // NoData windows take NoData whatever the code
-        if (__builtin_expect(__builtin_amdgcn_ballot_w64(amb != 0) != 0, 0)) {
+        // NoData windows take NoData whatever their candidates
+        if (__builtin_expect((ndflags >> 8) != 0, 0)) {
+            popmax = 0;
+#pragma unroll
+            for (int j = 0; j < CPT; j++)
+                if (!((ndflags >> (j + 8)) & 1u)) popmax = max(popmax, (unsigned)__builtin_popcount(cd[j]));
+        }
+        amb = popmax > 1 || band < kD8BandHalf;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(amb) != 0, 0)) {
 #pragma unroll
             for (int j = 0; j < CPT; j++) {
                 const int i = j + 1;
                 const unsigned c64 = d8_window_fp64(up.z[i - 1], up.z[i], up.z[i + 1], mid.z[i - 1], mid.z[i],
                                                     mid.z[i + 1], dn.z[i - 1], dn.z[i], dn.z[i + 1]);
-                if ((amb >> j) & 1u) cd[j] = c64;
+                if (amb) cd[j] = c64;
             }
         }
     }
-#endif
     if (DO_SLOPE && p.unit == PARGO_UNIT_DEGREES) {
 #pragma unroll
         for (int j = 0; j < CPT; j++) sl[j] = slope_unit(sl[j], p.unit);
@@ -554,21 +572,12 @@ template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
 __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
 {
     if (DO_D8) {
-#if PARGO_D8_F32
 #pragma unroll
         for (int j = 0; j < CPT; j++) up.V[j] = up.zf[j + 1] - below.zf[j + 1];
 #pragma unroll
         for (int i = 0; i <= CPT; i++) up.SE[i] = (up.zf[i] - below.zf[i + 1]) * kInvSqrt2f;
 #pragma unroll
         for (int i = 1; i <= CPT + 1; i++) up.SW[i] = (up.zf[i] - below.zf[i - 1]) * kInvSqrt2f;
-#else
-#pragma unroll
-        for (int j = 0; j < CPT; j++) up.V[j] = up.z[j + 1] - below.z[j + 1];
-#pragma unroll
-        for (int i = 0; i <= CPT; i++) up.SE[i] = div_sqrt2(up.z[i] - below.z[i + 1]);
-#pragma unroll
-        for (int i = 1; i <= CPT + 1; i++) up.SW[i] = div_sqrt2(up.z[i] - below.z[i - 1]);
-#endif
     }
 }
 

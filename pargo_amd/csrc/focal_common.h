@@ -16,12 +16,12 @@ constexpr double kD8Eps = 0.000001;
 
 // FP32 D8 filter (focal.hip): RN32(1/sqrt 2) and the thresholds that keep
 // the filter's decisions provably those of the FP64 tests.
-#ifndef PARGO_D8_F32
-#define PARGO_D8_F32 1
-#endif
 constexpr float kInvSqrt2f = 0x1.6a09e6p-1f;
-constexpr float kD8FlatF = 0.9999e-6f;
-constexpr float kD8SureF = 1.0001e-6f;
+constexpr float kD8SureF = 1.0001e-6f;        // K
+constexpr float kD8Shrink = 0x1.ffffep-1f;    // 1 - 2^-20
+constexpr float kD8FloorF = 0x1.0c6898p-20f;  // the float just below RN32(0.9999e-6)
+constexpr float kD8BandMid = 1.5e-6f;         // |M - 1.5e-6| < 0.50025e-6: ambiguous
+constexpr float kD8BandHalf = 0.50025e-6f;
 
 // RN(x / kSqrt2) for finite x derived from float32 data (|x| <= 2^129,
 // x = 0 or |x| >= 2^-149, so nothing below underflows).
