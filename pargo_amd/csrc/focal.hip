@@ -37,6 +37,8 @@
 #include <vector>
 
 #include "focal_common.h"
+
+
 #include "internal.h"
 
 namespace pargo {
@@ -120,18 +122,18 @@ __device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, in
     const int rr = r < 0 ? 0 : (r >= p.rows ? p.rows - 1 : r);
     const float *rowp = p.dem + (long long)rr * p.pitch;
     if constexpr (VECLOAD && CPT == 4) {
-        float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
+        float4 t = *reinterpret_cast<const float4 *>(rowp + (unsigned)c0);
         o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
     } else if constexpr (VECLOAD && CPT == 2) {
-        float2 t = *reinterpret_cast<const float2 *>(rowp + c0);
+        float2 t = *reinterpret_cast<const float2 *>(rowp + (unsigned)c0);
         o.v[0] = t.x; o.v[1] = t.y;
     } else {
 #pragma unroll
-        for (int k = 0; k < CPT; k++) o.v[k] = rowp[min(c0 + k, p.cols - 1)];
+        for (int k = 0; k < CPT; k++) o.v[k] = rowp[(unsigned)min(c0 + k, p.cols - 1)];
     }
     int hc = (lane == 0) ? c0 - 1 : (lane == 63 ? c0 + CPT : c0);
     hc = hc < 0 ? 0 : (hc > p.cols - 1 ? p.cols - 1 : hc);
-    o.hl = rowp[hc];
+    o.hl = rowp[(unsigned)hc];
 }
 
 // Cross-lane moves by DPP wavefront shifts (lane i reads lane i-1 / i+1).
@@ -142,6 +144,22 @@ __device__ __forceinline__ float from_lower_lane(float v)
 __device__ __forceinline__ float from_upper_lane(float v)
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// OR of v over each aligned group of N lanes (N = 4, 8 or 16), every lane
+// of the group getting the result: two quad permutations (lane ^ 1,
+// lane ^ 2), then the half-row mirror (lane i <-> 7 - i within 8 lanes)
+// and the row mirror (i <-> 15 - i within 16), each pairing a lane with one
+// of the other quad / half-row.
+template <int N>
+__device__ __forceinline__ unsigned or_lanes(unsigned v)
+{
+    static_assert(N == 4 || N == 8 || N == 16, "group of 4, 8 or 16 lanes");
+    v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    if (N >= 8) v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    if (N >= 16) v |= (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false); // row_mirror
+    return v;
 }
 
 // FP64 max/min without the quieting "v_max x,x" LLVM inserts for values it
@@ -222,7 +240,7 @@ __device__ __forceinline__ unsigned push_bit(unsigned c, bool cond)
 // slots and rotates their roles statically, so nothing is copied per row.
 template <int CPT>
 struct Slot {
-    double z[CPT + 2];   // the row
+    double z[CPT + 2];   // the row widened to FP64 (non-Horn slope methods only)
     float zf[CPT + 2];   // the row as loaded (float32)
     float V[CPT];        // RN32(z[j+1] - below[j+1])
     float SE[CPT + 1];   // RN32(RN32(z[i] - below[i+1]) * RN32(1/sqrt 2)), i = 0..CPT
@@ -240,7 +258,7 @@ struct Slot {
 // filter and the FP64 tests skip NaN drops exactly as the reference's > and
 // < comparisons do, and NaN is never NoData.  A signalling NaN turns mn/mx
 // into NaN, which only sends the row to the per-position test.
-template <int CPT>
+template <int CPT, bool KEEP_Z>
 __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CPT> &raw, int lane, Slot<CPT> &o)
 {
     float f[CPT + 2];
@@ -270,8 +288,10 @@ __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CP
         }
         o.flags = fl;
     }
+    if (KEEP_Z) {
 #pragma unroll
-    for (int i = 0; i < CPT + 2; i++) o.z[i] = (double)f[i];
+        for (int i = 0; i < CPT + 2; i++) o.z[i] = (double)f[i];
+    }
 #pragma unroll
     for (int i = 0; i < CPT + 2; i++) o.zf[i] = f[i];
 }
@@ -370,12 +390,12 @@ __device__ __forceinline__ float slope_unit(float s, int unit)
 // Output row r with up = row r-1, mid = row r, dn = the slot that receives
 // row r+1 from `raw`; `raw` is then refilled with row r+4.  `special` is set
 // when a window of this lane holds a value the fast path does not handle.
-template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK>
 __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, int lane, unsigned colb,
                                            Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw,
                                            bool &special)
 {
-    finish_row<CPT>(p, raw, lane, dn);
+    finish_row<CPT, DO_SLOPE && !HORN>(p, raw, lane, dn);
     load_raw<CPT, VECLOAD>(p, r + 4, c0, lane, raw);
 
     float sl[CPT];
@@ -389,8 +409,9 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
             double col[CPT + 2], vd[CPT + 2];
 #pragma unroll
             for (int i = 0; i < CPT + 2; i++) {
-                col[i] = __builtin_fma(2.0, mid.z[i], dn.z[i]) + up.z[i];
-                vd[i] = up.z[i] - dn.z[i];
+                const double u = up.zf[i], m = mid.zf[i], d = dn.zf[i];
+                col[i] = __builtin_fma(2.0, m, d) + u;
+                vd[i] = u - d;
             }
 #pragma unroll
             for (int j = 0; j < CPT; j++) {
@@ -497,8 +518,8 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
 #pragma unroll
             for (int j = 0; j < CPT; j++) {
                 const int i = j + 1;
-                const unsigned c64 = d8_window_fp64(up.z[i - 1], up.z[i], up.z[i + 1], mid.z[i - 1], mid.z[i],
-                                                    mid.z[i + 1], dn.z[i - 1], dn.z[i], dn.z[i + 1]);
+                const unsigned c64 = d8_window_fp64(up.zf[i - 1], up.zf[i], up.zf[i + 1], mid.zf[i - 1], mid.zf[i],
+                                                    mid.zf[i + 1], dn.zf[i - 1], dn.zf[i], dn.zf[i + 1]);
                 if (amb) cd[j] = c64;
             }
         }
@@ -524,46 +545,58 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, 
     }
 
     // ---- stores --------------------------------------------------------------
-    const long long base = (long long)r * p.pitch + c0;
+    const long long rbase = (long long)r * p.pitch;
+    const unsigned uc0 = (unsigned)c0;  // column indexes are never negative
     if (DO_SLOPE) {
+        float *srow = p.slope + rbase;
         if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
-                *reinterpret_cast<float4 *>(p.slope + base) = make_float4(sl[0], sl[1], sl[2], sl[3]);
+                *reinterpret_cast<float4 *>(srow + uc0) = make_float4(sl[0], sl[1], sl[2], sl[3]);
             } else if constexpr (CPT == 2) {
-                *reinterpret_cast<float2 *>(p.slope + base) = make_float2(sl[0], sl[1]);
+                *reinterpret_cast<float2 *>(srow + uc0) = make_float2(sl[0], sl[1]);
             } else {
 #pragma unroll
-                for (int j = 0; j < CPT; j++) p.slope[base + j] = sl[j];
+                for (int j = 0; j < CPT; j++) srow[uc0 + j] = sl[j];
             }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; j++)
-                if (c0 + j < p.cols) p.slope[base + j] = sl[j];
+                if (c0 + j < p.cols) srow[uc0 + j] = sl[j];
         }
     }
     if (DO_D8) {
+        uint8_t *crow = p.code + rbase;
         if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
-                *reinterpret_cast<unsigned *>(p.code + base) = cd[0] | (cd[1] << 8) | (cd[2] << 16) | (cd[3] << 24);
+                *reinterpret_cast<unsigned *>(crow + uc0) = cd[0] | (cd[1] << 8) | (cd[2] << 16) | (cd[3] << 24);
             } else if constexpr (CPT == 2) {
-                *reinterpret_cast<unsigned short *>(p.code + base) = (unsigned short)(cd[0] | (cd[1] << 8));
+                *reinterpret_cast<unsigned short *>(crow + uc0) = (unsigned short)(cd[0] | (cd[1] << 8));
             } else {
 #pragma unroll
-                for (int j = 0; j < CPT; j++) p.code[base + j] = (uint8_t)cd[j];
+                for (int j = 0; j < CPT; j++) crow[uc0 + j] = (uint8_t)cd[j];
             }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; j++)
-                if (c0 + j < p.cols) p.code[base + j] = (uint8_t)cd[j];
+                if (c0 + j < p.cols) crow[uc0 + j] = (uint8_t)cd[j];
         }
     }
-    if (p.mask) {
-        // gather the CPT-bit groups of 32/CPT lanes into one word
+    if (MASK && p.mask) {
+        // gather the CPT-bit groups of 32/CPT lanes into one word (nothing
+        // to gather when no lane of the wave has a NoData cell in this row)
+        // The lane's shift is re-derived from the lane id where it is used:
+        // a per-lane value kept across the row loop can end up spilled, and
+        // its reload waits for every load in flight, the row prefetches
+        // included.
         constexpr int LPW = 32 / CPT;
-        unsigned w = ndbits << (CPT * (lane % LPW));
-#pragma unroll
-        for (int o = 1; o < LPW; o <<= 1) w |= __shfl_xor(w, o, 64);
-        if ((lane % LPW) == 0 && c0 < p.cols) p.mask[(long long)r * p.mask_words + (c0 >> 5)] = w;
+        unsigned w = 0;
+        if (__builtin_amdgcn_ballot_w64(ndbits != 0) != 0) {
+            unsigned ln;
+            asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(ln));
+            w = or_lanes<LPW>(ndbits << (CPT * (ln % LPW)));
+        }
+        uint32_t *mrow = p.mask + (long long)r * p.mask_words;
+        if ((lane % LPW) == 0 && c0 < p.cols) mrow[uc0 >> 5] = w;
     }
 }
 
@@ -583,7 +616,7 @@ __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
 
 // One wave computes output rows [rbeg, rend) of the 64*CPT columns that
 // start at wave_c0 (wave-uniform).
-template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK>
 __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
 {
     const int c0 = wave_c0 + lane * CPT;
@@ -600,18 +633,18 @@ __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_
         load_raw<CPT, VECLOAD>(p, rbeg + 1, c0, lane, R0);
         load_raw<CPT, VECLOAD>(p, rbeg + 2, c0, lane, R1);
         load_raw<CPT, VECLOAD>(p, rbeg + 3, c0, lane, R2);
-        finish_row<CPT>(p, ra, lane, A);
-        finish_row<CPT>(p, rb, lane, B);
+        finish_row<CPT, DO_SLOPE && !HORN>(p, ra, lane, A);
+        finish_row<CPT, DO_SLOPE && !HORN>(p, rb, lane, B);
     }
     pair_rows<CPT, DO_SLOPE, DO_D8, HORN>(A, B);
     bool special = false;
     // Three slots rotate roles statically: (up, mid, dn) = (A,B,C), (B,C,A), (C,A,B).
     for (int r = rbeg; r < rend; r += 3) {
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN>(p, r, c0, lane, colb, A, B, C, R0, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r, c0, lane, colb, A, B, C, R0, special);
         if (r + 1 >= rend) break;
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN>(p, r + 1, c0, lane, colb, B, C, A, R1, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 1, c0, lane, colb, B, C, A, R1, special);
         if (r + 2 >= rend) break;
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN>(p, r + 2, c0, lane, colb, C, A, B, R2, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 2, c0, lane, colb, C, A, B, R2, special);
     }
     // queue the strip for the exact pass (rare: Inf or |z| >= 2^40 nearby)
     if (__builtin_amdgcn_ballot_w64(special) != 0 && lane == 0) {
@@ -622,20 +655,20 @@ __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_
 
 // VEC: the raster's pitch and base allow vector access; a wave uses it when
 // all its columns are inside the raster.
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK = true>
 __device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
 {
     if constexpr (VEC) {
         if (wave_c0 + 64 * CPT <= p.cols) {
-            focal_strip_impl<CPT, true, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
+            focal_strip_impl<CPT, true, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
             return;
         }
     }
-    focal_strip_impl<CPT, false, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
+    focal_strip_impl<CPT, false, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
 }
 
 // One raster: block = 4 waves side by side, one row strip per block.
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK = true>
 __device__ __forceinline__ void focal_raster(const FocalParams &p)
 {
     const int lane = threadIdx.x & 63;
@@ -643,7 +676,7 @@ __device__ __forceinline__ void focal_raster(const FocalParams &p)
     const int wave_c0 = (blockIdx.x * 4 + wave) * 64 * CPT;
     const int rbeg = blockIdx.y * p.rows_per_strip;
     const int rend = min(rbeg + p.rows_per_strip, p.rows);
-    focal_strip<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p, wave_c0, lane, rbeg, rend);
+    focal_strip<CPT, VEC, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
 }
 
 template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
@@ -652,14 +685,24 @@ __global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
     focal_raster<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p);
 }
 
-// The fused Horn slope + D8 pass at four waves per SIMD (at most 128 VGPRs,
-// a few spilled to scratch): it ran faster than at three waves without
-// spills, since the pass issues VALU work most of the time and a fourth
-// wave fills the dependency stalls.  The other variants keep three.
+// The fused Horn slope + D8 pass without a NoData mask, at four waves per
+// SIMD (at most 128 VGPRs, a few spilled to scratch): it ran faster than at
+// three waves without spills, since the pass issues VALU work most of the
+// time and a fourth wave fills the dependency stalls.  That balance is
+// fragile: any extra state in the loop (the mask's, for one) brings spill
+// reloads into it, and each reload waits for every load in flight.  The
+// variant that writes the mask therefore runs at three waves, like the
+// other kernels.
 template <int CPT, bool VEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void focal_kernel_fused(FocalParams p)
 {
-    focal_raster<CPT, VEC, true, true, true>(p);
+    focal_raster<CPT, VEC, true, true, true, false>(p);
+}
+
+template <int CPT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void focal_kernel_fused_mask(FocalParams p)
+{
+    focal_raster<CPT, VEC, true, true, true, true>(p);
 }
 
 // Many rasters in one persistent launch: every wave takes (tile, column
@@ -774,8 +817,13 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
     dim3 grid((p.cols + cols_per_block - 1) / cols_per_block, (p.rows + cfg.rows_per_strip - 1) / cfg.rows_per_strip);
     if (cfg.cpt == 4) {
         if constexpr (DO_SLOPE && DO_D8 && HORN) {
-            if (vec) hipLaunchKernelGGL((focal_kernel_fused<4, true>), grid, block, 0, s, p);
-            else hipLaunchKernelGGL((focal_kernel_fused<4, false>), grid, block, 0, s, p);
+            if (p.mask) {
+                if (vec) hipLaunchKernelGGL((focal_kernel_fused_mask<4, true>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((focal_kernel_fused_mask<4, false>), grid, block, 0, s, p);
+            } else {
+                if (vec) hipLaunchKernelGGL((focal_kernel_fused<4, true>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((focal_kernel_fused<4, false>), grid, block, 0, s, p);
+            }
         } else {
             if (vec) hipLaunchKernelGGL((focal_kernel<4, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
             else hipLaunchKernelGGL((focal_kernel<4, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
