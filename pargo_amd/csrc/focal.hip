@@ -388,15 +388,18 @@ __device__ __forceinline__ float slope_unit(float s, int unit)
 }
 
 // Output row r with up = row r-1, mid = row r, dn = the slot that receives
-// row r+1 from `raw`; `raw` is then refilled with row r+4.  `special` is set
-// when a window of this lane holds a value the fast path does not handle.
+// row r+1 from `raw`; `raw` is then refilled with row r+4, or with row
+// rlast (the strip's last input row) once r+4 passes it, so that a strip
+// fetches no row of the next one (a repeated row is a cache hit).
+// `special` is set when a window of this lane holds a value the fast path
+// does not handle.
 template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK>
-__device__ __forceinline__ void focal_step(const FocalParams &p, int r, int c0, int lane, unsigned colb,
+__device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlast, int c0, int lane, unsigned colb,
                                            Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw,
                                            bool &special)
 {
     finish_row<CPT, DO_SLOPE && !HORN>(p, raw, lane, dn);
-    load_raw<CPT, VECLOAD>(p, r + 4, c0, lane, raw);
+    load_raw<CPT, VECLOAD>(p, min(r + 4, rlast), c0, lane, raw);
 
     float sl[CPT];
     unsigned cd[CPT];
@@ -630,9 +633,9 @@ __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_
         RawRow<CPT> ra, rb;
         load_raw<CPT, VECLOAD>(p, rbeg - 1, c0, lane, ra);
         load_raw<CPT, VECLOAD>(p, rbeg, c0, lane, rb);
-        load_raw<CPT, VECLOAD>(p, rbeg + 1, c0, lane, R0);
-        load_raw<CPT, VECLOAD>(p, rbeg + 2, c0, lane, R1);
-        load_raw<CPT, VECLOAD>(p, rbeg + 3, c0, lane, R2);
+        load_raw<CPT, VECLOAD>(p, min(rbeg + 1, rend), c0, lane, R0);
+        load_raw<CPT, VECLOAD>(p, min(rbeg + 2, rend), c0, lane, R1);
+        load_raw<CPT, VECLOAD>(p, min(rbeg + 3, rend), c0, lane, R2);
         finish_row<CPT, DO_SLOPE && !HORN>(p, ra, lane, A);
         finish_row<CPT, DO_SLOPE && !HORN>(p, rb, lane, B);
     }
@@ -640,11 +643,11 @@ __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_
     bool special = false;
     // Three slots rotate roles statically: (up, mid, dn) = (A,B,C), (B,C,A), (C,A,B).
     for (int r = rbeg; r < rend; r += 3) {
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r, c0, lane, colb, A, B, C, R0, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r, rend, c0, lane, colb, A, B, C, R0, special);
         if (r + 1 >= rend) break;
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 1, c0, lane, colb, B, C, A, R1, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 1, rend, c0, lane, colb, B, C, A, R1, special);
         if (r + 2 >= rend) break;
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 2, c0, lane, colb, C, A, B, R2, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 2, rend, c0, lane, colb, C, A, B, R2, special);
     }
     // queue the strip for the exact pass (rare: Inf or |z| >= 2^40 nearby)
     if (__builtin_amdgcn_ballot_w64(special) != 0 && lane == 0) {
