@@ -13,22 +13,32 @@
 //   mask            idwOperator.cpp:446-461, gpro/rasterLayer.h:1296-1316
 //
 // Device layout: samples bucketed by block into CSR order (file order inside
-// a block), x/y interleaved as double2 so one candidate is one 16-byte load,
-// values and the file index of each bucket slot in separate arrays.
+// a block), each bucket padded to a multiple of 4 slots with NaN samples
+// (which never pass a distance test), x/y interleaved as double2; values
+// and the file index of each bucket slot in separate arrays, the real
+// sample count of each block beside the padded starts.
 //
-// Work split: one lane per output cell, a wave covers an 8x8 cell tile, so
-// its 64 lanes almost always share one sample block and walk the same ring
-// and block sequence (the loads of a candidate hit one address).  Lanes keep
-// their K nearest (distance, bucket slot) pairs in registers.
+// Work split: one lane per output cell, a wave covers an 8x8 cell tile.
+// The lanes of a wave almost always share one sample block, so the wave
+// walks the ring sequence of one centre block at a time (once in the common
+// case, twice or four times for a tile on a block edge or corner), every
+// lane with that centre taking part: candidate addresses are wave-uniform,
+// so candidates arrive through scalar loads, four at a time, and each lane
+// only does its distance test.  A lane whose own search has ended (ring
+// stop, buffer stop) stays in the walk with a threshold no distance passes.
+// Lanes keep their K nearest (distance, bucket slot) pairs in registers.
 //
 // Exactness: squared distances are formed as RN(RN(dx*dx) + RN(dy*dy)) from
 // dx = RN(xs - cx), the operands of the reference's sqrt(pow(dx,2) +
-// pow(dy,2)).  A candidate is pre-filtered by s <= maxDist^2 * (1 + 2^-40),
-// a necessary condition for tmpDist < maxDist; survivors get the correctly
-// rounded FP64 sqrt and the reference's exact comparisons, so the K-set and
-// its slot order equal the reference's.
+// pow(dy,2)).  A candidate is pre-filtered by an FMA form of its squared
+// distance against maxDist^2 * (1 + 2^-40), a necessary condition for
+// tmpDist < maxDist; survivors get the reference's squared distance, the
+// correctly rounded FP64 sqrt and the reference's exact comparisons, so the
+// K-set and its slot order equal the reference's.  A block whose samples'
+// bounding box is too far for every lane is skipped without changing either.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "internal.h"
@@ -44,8 +54,11 @@ struct pargo_idw_plan {
     void *d_mem = nullptr;
     double2 *d_xy = nullptr;   // bucket order
     double *d_v = nullptr;     // bucket order
-    int *d_start = nullptr;    // blockRows*blockCols + 1
-    int *d_perm = nullptr;     // bucket slot -> file index
+    int *d_start = nullptr;    // blockRows*blockCols + 1, padded slot offsets
+    int *d_count = nullptr;    // blockRows*blockCols, samples per block
+    double4 *d_bbox = nullptr; // blockRows*blockCols, (min x, max x, min y, max y) of the samples
+    int *d_perm = nullptr;     // bucket slot -> file index (-1: padding)
+    int slots = 0;             // padded slot count
     std::vector<int> h_perm;
 };
 
@@ -79,6 +92,8 @@ struct IdwParams {
     const double2 *__restrict__ xy;
     const double *__restrict__ v;
     const int *__restrict__ start;
+    const int *__restrict__ count;
+    const double4 *__restrict__ bbox;
     const int32_t *__restrict__ mask;
     float *__restrict__ out;             // may be null (work only)
     double *__restrict__ work_out;       // per cell: samples examined, may be null
@@ -118,15 +133,43 @@ __device__ __forceinline__ double powi(double d, int p)
     return p < 0 ? 1.0 / r : r;
 }
 
-// The K-nearest search of one cell.  Fills dist[]/slot[] like nbrSamples
-// and returns min(found, K).  Counts distance evaluations in *ncand.
-template <int MAXK>
-__device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, double cellY, double dist[MAXK],
-                                          int slot[MAXK], unsigned &ncand)
+// Eight candidates (128 bytes at a wave-uniform address) as two scalar
+// loads in flight together and one wait for both.
+typedef unsigned int u32x16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ void load8_uniform(const double2 *src, double2 c[8])
+{
+    // the address is wave-uniform; say so, so that it sits in SGPRs
+    const unsigned long long a64 = (unsigned long long)src;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a64);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a64 >> 32));
+    const unsigned long long base = ((unsigned long long)hi << 32) | lo;
+    u32x16 a, b;
+    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx16 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+                 : "=s"(a), "=s"(b)
+                 : "s"(base)
+                 : "memory");
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        c[i].x = __hiloint2double((int)a[4 * i + 1], (int)a[4 * i]);
+        c[i].y = __hiloint2double((int)a[4 * i + 3], (int)a[4 * i + 2]);
+        c[i + 4].x = __hiloint2double((int)b[4 * i + 1], (int)b[4 * i]);
+        c[i + 4].y = __hiloint2double((int)b[4 * i + 3], (int)b[4 * i + 2]);
+    }
+}
+
+// The K-nearest search of the lanes whose cell lies in sample block
+// (blockRow, blockCol) (wave-uniform) and that have `mine` set.  Each lane's
+// list of (distance, bucket slot) pairs lives in LDS at kd[i * 256] /
+// ks[i * 256], i < K (the caller offsets the pointers by the lane's thread
+// index): the list is written on an insertion and read on a replacement's
+// rescan and at the end, while the test of a candidate needs only the
+// running threshold in registers.  Fills the list like nbrSamples and
+// returns min(found, K).  Counts the samples of the visited blocks in
+// *ncand.
+__device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow, int blockCol, double cellX,
+                                                double cellY, bool mine, double *kd, int *ks, unsigned &ncand)
 {
     const int K = p.k;
-    const int blockRow = (int)((p.maxY - cellY) / p.block);
-    const int blockCol = (int)((cellX - p.minX) / p.block);
     // getMinDistanceToBlockBound (idwOperator.cpp:5-20), one rank
     const double bx0 = blockCol * p.block + p.minX, bx1 = (blockCol + 1) * p.block + p.minX;
     const double by0 = p.maxY - blockRow * p.block, by1 = p.maxY - (blockRow + 1) * p.block;
@@ -137,16 +180,58 @@ __device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, doub
     double maxDist = 0.0;
     int maxIdx = -1;
     int tail = -1;
-    double kthr = INFINITY;  // prefilter on squared distance while filling
-#pragma unroll
-    for (int i = 0; i < MAXK; i++) {
-        dist[i] = 0.0;
-        slot[i] = 0;
-    }
+    bool act = mine;
+    // prefilter on squared distance: everything passes while the list fills,
+    // nothing once this lane's search has ended
+    double kthr = act ? INFINITY : -INFINITY;
+    // The prefilter's squared distance takes one FMA: it is within 2^-50
+    // (relative) of the reference's RN(RN(dx^2) + RN(dy^2)), far inside the
+    // 2^-40 slack of kthr, so it still passes every candidate the exact
+    // test can take; that test recomputes the reference's value.
+    auto sqdist = [&](double2 xy) {
+        const double dx = xy.x - cellX, dy = xy.y - cellY;
+        return __builtin_fma(dx, dx, dy * dy);
+    };
+    auto visit = [&](double2 xy, double pre, int q) {
+        if (pre <= kthr) {
+            const double dx = xy.x - cellX, dy = xy.y - cellY;
+            const double s = dx * dx + dy * dy;
+            const double tmpDist = (s == 0.0) ? kIdwEps : sqrt(s);
+            if (tail < K - 1) {
+                tail++;
+                kd[tail * 256] = tmpDist;
+                ks[tail * 256] = q;
+                if (tmpDist > maxDist) {
+                    maxDist = tmpDist;
+                    maxIdx = tail;
+                }
+                if (tail == K - 1) kthr = maxDist * maxDist * kPrefilterSlack;
+            } else if (tmpDist < maxDist) {
+                // (the reference also counts tailIdx up here; past K-1 only
+                // "tail >= K-1" is ever used)
+                kd[maxIdx * 256] = tmpDist;
+                ks[maxIdx * 256] = q;
+                maxDist = kd[0];
+                maxIdx = 0;
+                for (int i = 1; i < K; i++) {
+                    const double di = kd[i * 256];
+                    if (di > maxDist) {
+                        maxDist = di;
+                        maxIdx = i;
+                    }
+                }
+                kthr = maxDist * maxDist * kPrefilterSlack;
+            }
+        }
+    };
     for (int rad = 0;; rad++) {
         const double searchRange = (double)(rad - 1) * p.block + minDistToBound;
-        if (p.buffer > 0 && searchRange >= p.buffer) break;
+        if (act && p.buffer > 0 && searchRange >= p.buffer) {
+            act = false;
+            kthr = -INFINITY;
+        }
         if (rad > p.maxRing) break;  // every further ring is outside the block grid
+        if (__builtin_amdgcn_ballot_w64(act) == 0) break;
         for (int tRow = blockRow - rad; tRow <= blockRow + rad; ++tRow) {
             if (tRow < 0 || tRow >= p.blockRows) continue;
             const bool full = (tRow == blockRow - rad) || (tRow == blockRow + rad);
@@ -156,57 +241,63 @@ __device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, doub
                 if (tCol < 0 || tCol >= p.blockCols) continue;
                 const int bi = tRow * p.blockCols + tCol;
                 const int s0 = p.start[bi], s1 = p.start[bi + 1];
-                ncand += (unsigned)(s1 - s0);
-                // one candidate ahead: the load of the next coordinates
-                // overlaps this candidate's distance test (the bucket
-                // array carries one padding entry after the last sample)
-                double2 nxt = p.xy[s0];
-                for (int q = s0; q < s1; q++) {
-                    const double2 xy = nxt;
-                    nxt = p.xy[q + 1];
-                    const double dx = xy.x - cellX, dy = xy.y - cellY;
-                    const double s = dx * dx + dy * dy;
-                    if (s <= kthr) {
-                        const double tmpDist = (s == 0.0) ? kIdwEps : sqrt(s);
-                        if (tail < K - 1) {
-                            tail++;
+                if (act) ncand += (unsigned)p.count[bi];
+                if (s0 == s1) continue;
+                // Skip the block when no lane can take any of its samples:
+                // RN is monotone, so RN(RN(dxb^2) + RN(dyb^2)) from the
+                // distances to the samples' bounding box is a lower bound of
+                // every sample's squared distance as computed below, and a
+                // sample above kthr is never taken (kthr is +inf while a
+                // lane's list fills and -inf once its search has ended).
+                {
+                    const double4 bb = p.bbox[bi];
+                    const double dxb = fmax(fmax(bb.x - cellX, cellX - bb.y), 0.0);
+                    const double dyb = fmax(fmax(bb.z - cellY, cellY - bb.w), 0.0);
+                    const double lb2 = dxb * dxb + dyb * dyb;
+                    if (__builtin_amdgcn_ballot_w64(lb2 <= kthr) == 0) continue;
+                }
+                // Eight candidates per scalar-load wait (scalar loads return
+                // out of order, so every wait is for all of them), then the
+                // bucket's last four if its padded length is not a multiple
+                // of eight.  Usually none passes once the list is full;
+                // otherwise they go in order (each may lower kthr).
+                int q = s0;
+                for (; q + 8 <= s1; q += 8) {
+                    double2 c[8];
+                    load8_uniform(p.xy + q, c);
+                    double e[8];
 #pragma unroll
-                            for (int i = 0; i < MAXK; i++)
-                                if (i == tail) {
-                                    dist[i] = tmpDist;
-                                    slot[i] = q;
-                                }
-                            if (tmpDist > maxDist) {
-                                maxDist = tmpDist;
-                                maxIdx = tail;
-                            }
-                            if (tail == K - 1) kthr = maxDist * maxDist * kPrefilterSlack;
-                        } else {
-                            // (the reference also counts tailIdx up here;
-                            // past K-1 only "tail >= K-1" is ever used)
-                            if (tmpDist < maxDist) {
+                    for (int i = 0; i < 8; i++) e[i] = sqdist(c[i]);
+                    bool any = false;
 #pragma unroll
-                                for (int i = 0; i < MAXK; i++)
-                                    if (i == maxIdx) {
-                                        dist[i] = tmpDist;
-                                        slot[i] = q;
-                                    }
-                                maxDist = dist[0];
-                                maxIdx = 0;
+                    for (int i = 0; i < 8; i++) any |= e[i] <= kthr;
+                    if (__builtin_amdgcn_ballot_w64(any) != 0) {
 #pragma unroll
-                                for (int i = 1; i < MAXK; i++)
-                                    if (i < K && dist[i] > maxDist) {
-                                        maxDist = dist[i];
-                                        maxIdx = i;
-                                    }
-                                kthr = maxDist * maxDist * kPrefilterSlack;
-                            }
-                        }
+                        for (int i = 0; i < 8; i++) visit(c[i], e[i], q + i);
+                    }
+                }
+                if (q < s1) {
+                    double2 c[4];
+                    double e[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        c[i] = p.xy[q + i];
+                        e[i] = sqdist(c[i]);
+                    }
+                    bool any = false;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) any |= e[i] <= kthr;
+                    if (__builtin_amdgcn_ballot_w64(any) != 0) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) visit(c[i], e[i], q + i);
                     }
                 }
             }
         }
-        if (tail >= K - 1 && searchRange >= maxDist) break;
+        if (act && tail >= K - 1 && searchRange >= maxDist) {
+            act = false;
+            kthr = -INFINITY;
+        }
     }
     return (tail + 1 < K) ? tail + 1 : K;
 }
@@ -217,7 +308,11 @@ __device__ __forceinline__ int idw_search(const IdwParams &p, double cellX, doub
 // an LDS hash table (open addressing on the bucket slot) and flushed with
 // one global FP64 atomic per distinct sample; a crowded table falls back to
 // global atomics.
-constexpr int kAdjTable = 1024;
+// 256 entries (3 KiB) keep the table and the K-lists of a workgroup within
+// 40 KiB of LDS, four workgroups per CU; a 16x16 tile selects a few dozen
+// distinct samples.
+constexpr int kAdjTableBits = 8;
+constexpr int kAdjTable = 1 << kAdjTableBits;
 constexpr int kAdjProbe = 16;
 
 template <int MAXK, bool ADJOINT>
@@ -225,6 +320,8 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
 {
     __shared__ int tkey[ADJOINT ? kAdjTable : 1];
     __shared__ double tval[ADJOINT ? kAdjTable : 1];
+    __shared__ double kdist[MAXK * 256];  // [slot i][thread]
+    __shared__ int kslot[MAXK * 256];
     if constexpr (ADJOINT) {
         for (int i = threadIdx.x; i < kAdjTable; i += blockDim.x) {
             tkey[i] = -1;
@@ -233,7 +330,7 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         __syncthreads();
     }
     auto adj_add = [&](int q, double v) {
-        unsigned h = ((unsigned)q * 2654435761u) >> 22;  // 10-bit hash
+        unsigned h = ((unsigned)q * 2654435761u) >> (32 - kAdjTableBits);
         for (int t = 0; t < kAdjProbe; t++, h = (h + 1) & (kAdjTable - 1)) {
             const int k = atomicCAS(&tkey[h], -1, q);
             if (k == -1 || k == q) {
@@ -246,49 +343,49 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int col = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const bool inside = row < p.rows && col < p.cols;
     unsigned ncand = 0;
-    if (row < p.rows && col < p.cols) {
-        float result;
-        bool skip = false;
-        if (p.mask) {
-            const int mr = mask_index(p.mask_rows, p.rows, row);
-            const int mc = mask_index(p.mask_cols, p.cols, col);
-            const int m = p.mask[(long long)mr * p.mask_cols + mc];
-            skip = (m == 0) | (m == p.mask_nodata);
-        }
-        if (skip) {
-            result = -9999.0f;
-        } else {
-            const double cellX = (double)((col + 0.5) * (double)p.cs + p.minX);
-            const double cellY = (double)(p.maxY - (row + 0.5) * (double)p.cs);
-            double dist[MAXK];
-            int slot[MAXK];
-            const int num = idw_search<MAXK>(p, cellX, cellY, dist, slot, ncand);
-            double wsum = 0.0, w[MAXK];
-#pragma unroll
-            for (int i = 0; i < MAXK; i++) {
-                w[i] = 0.0;
-                if (i < num) {
-                    w[i] = 1.0 / powi(dist[i], p.power);
-                    wsum += w[i];
-                }
-            }
+    float result = -9999.0f;
+    bool pending = inside;
+    if (inside && p.mask) {
+        const int mr = mask_index(p.mask_rows, p.rows, row);
+        const int mc = mask_index(p.mask_cols, p.cols, col);
+        const int m = p.mask[(long long)mr * p.mask_cols + mc];
+        pending = !((m == 0) | (m == p.mask_nodata));
+    }
+    const double cellX = (double)((col + 0.5) * (double)p.cs + p.minX);
+    const double cellY = (double)(p.maxY - (row + 0.5) * (double)p.cs);
+    const int blockRow = (int)((p.maxY - cellY) / p.block);
+    const int blockCol = (int)((cellX - p.minX) / p.block);
+    // one search per distinct centre block among the pending lanes
+    for (;;) {
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(pending);
+        if (m == 0) break;
+        const int leader = (int)__builtin_ctzll(m);
+        const int gRow = __builtin_amdgcn_readlane(blockRow, leader);
+        const int gCol = __builtin_amdgcn_readlane(blockCol, leader);
+        const bool mine = pending && blockRow == gRow && blockCol == gCol;
+        double *kd = kdist + threadIdx.x;
+        int *ks = kslot + threadIdx.x;
+        const int num = idw_search_group(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand);
+        if (mine) {
+            // weights in slot order (idwOperator.cpp:472-480)
+            double wsum = 0.0;
+            for (int i = 0; i < num; i++) wsum += 1.0 / powi(kd[i * 256], p.power);
             if (ADJOINT) {
                 const double gc = (double)p.grad_out[(long long)row * p.cols + col];
-#pragma unroll
-                for (int i = 0; i < MAXK; i++)
-                    if (i < num && gc != 0.0) adj_add(slot[i], gc * w[i] / wsum);
+                if (gc != 0.0)
+                    for (int i = 0; i < num; i++) adj_add(ks[i * 256], gc * (1.0 / powi(kd[i * 256], p.power)) / wsum);
             } else {
                 double acc = 0.0;
-#pragma unroll
-                for (int i = 0; i < MAXK; i++)
-                    if (i < num) acc += p.v[slot[i]] * w[i] / wsum;
+                for (int i = 0; i < num; i++) acc += p.v[ks[i * 256]] * (1.0 / powi(kd[i * 256], p.power)) / wsum;
                 result = (float)acc;
             }
         }
-        if (!ADJOINT && p.out) p.out[(long long)row * p.cols + col] = result;
-        if (!ADJOINT && p.work_out) p.work_out[(long long)row * p.cols + col] = (double)ncand;
+        pending = pending && !mine;
     }
+    if (!ADJOINT && inside && p.out) p.out[(long long)row * p.cols + col] = result;
+    if (!ADJOINT && inside && p.work_out) p.work_out[(long long)row * p.cols + col] = (double)ncand;
     if constexpr (ADJOINT) {
         __syncthreads();
         for (int i = threadIdx.x; i < kAdjTable; i += blockDim.x) {
@@ -323,12 +420,12 @@ static int launch_idw(const IdwParams &p, hipStream_t s)
     return PARGO_OK;
 }
 
-// grad_vals[file index] = grad_acc[bucket slot]
-__global__ void idw_unbucket_kernel(const double *__restrict__ acc, const int *__restrict__ perm, int n,
+// grad_vals[file index] = grad_acc[bucket slot] (padding slots skipped)
+__global__ void idw_unbucket_kernel(const double *__restrict__ acc, const int *__restrict__ perm, int slots,
                                     double *__restrict__ out)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) out[perm[q]] = acc[q];
+    if (q < slots && perm[q] >= 0) out[perm[q]] = acc[q];
 }
 
 // Compute-load estimate of one coarse cell (IdwTransformation::Operator,
@@ -337,7 +434,7 @@ __global__ void idw_unbucket_kernel(const double *__restrict__ acc, const int *_
 // visited (the centre block again at r = 0), until K samples are counted.
 // load = blocks visited + 2 * samples counted.
 struct IdwLoadParams {
-    const int *__restrict__ start;
+    const int *__restrict__ count;
     const int32_t *__restrict__ mask;
     double *__restrict__ out;
     unsigned *__restrict__ nonterm;
@@ -364,7 +461,7 @@ __global__ __launch_bounds__(256) void idw_load_kernel(IdwLoadParams p)
     // getBlockRowIndexByCellIndex(iRow, cellSize) (idwOperator.h:70-71)
     const int bR = min((int)((cRow + 0.5) * p.ccs / p.block), p.blockRows - 1);
     const int bC = min((int)((cCol + 0.5) * p.ccs / p.block), p.blockCols - 1);
-    auto count = [&](int r, int c) { return p.start[r * p.blockCols + c + 1] - p.start[r * p.blockCols + c]; };
+    auto count = [&](int r, int c) { return p.count[r * p.blockCols + c]; };
     long long nbr = count(bR, bC);
     long long blocks = 0;
     bool enough = false;
@@ -405,6 +502,8 @@ int idw_fill_params(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power
     p.xy = plan->d_xy;
     p.v = plan->d_v;
     p.start = plan->d_start;
+    p.count = plan->d_count;
+    p.bbox = plan->d_bbox;
     p.mask = mask;
     p.minX = plan->minX;
     p.maxY = plan->maxY;
@@ -494,21 +593,37 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
         bidx[i] = (int)index;
         start[index + 1]++;
     }
-    for (int b = 0; b < nb; b++) start[b + 1] += start[b];
+    // counts, then slot offsets with every bucket padded to a multiple of 4
+    std::vector<int> count(nb);
+    for (int b = 0; b < nb; b++) count[b] = start[b + 1];
+    start[0] = 0;
+    for (int b = 0; b < nb; b++) start[b + 1] = start[b] + ((count[b] + 3) & ~3);
+    const int slots = start[nb];
+    pl->slots = slots;
     std::vector<int> fill(start.begin(), start.end() - 1);
-    std::vector<double2> xy((size_t)n + 1, make_double2(0.0, 0.0));
-    std::vector<double> v(n);
-    pl->h_perm.resize(n);
+    // 4 trailing padding slots: a lane may read one candidate past its bucket
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    std::vector<double2> xy((size_t)slots + 4, make_double2(nan, nan));
+    std::vector<double> v((size_t)slots, 0.0);
+    pl->h_perm.assign((size_t)slots, -1);
+    const double inf = std::numeric_limits<double>::infinity();
+    std::vector<double4> bbox((size_t)nb, make_double4(inf, -inf, inf, -inf));
     for (int i = 0; i < n; i++) {
         const int q = fill[bidx[i]]++;
         xy[q] = make_double2(xs[i], ys[i]);
         v[q] = vals[i];
         pl->h_perm[q] = i;
+        double4 &bb = bbox[bidx[i]];
+        bb.x = std::min(bb.x, xs[i]);
+        bb.y = std::max(bb.y, xs[i]);
+        bb.z = std::min(bb.z, ys[i]);
+        bb.w = std::max(bb.w, ys[i]);
     }
-    const size_t bytes_xy = sizeof(double2) * ((size_t)n + 1), bytes_v = sizeof(double) * n;
-    const size_t bytes_start = sizeof(int) * (nb + 1), bytes_perm = sizeof(int) * n;
+    const size_t bytes_xy = sizeof(double2) * ((size_t)slots + 4), bytes_v = sizeof(double) * slots;
+    const size_t bytes_start = sizeof(int) * (nb + 1), bytes_count = sizeof(int) * nb, bytes_perm = sizeof(int) * slots;
+    const size_t bytes_bbox = sizeof(double4) * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t total = al(bytes_xy) + al(bytes_v) + al(bytes_start) + al(bytes_perm);
+    const size_t total = al(bytes_xy) + al(bytes_v) + al(bytes_start) + al(bytes_count) + al(bytes_perm) + al(bytes_bbox);
     hipError_t e = hipMalloc(&pl->d_mem, total);
     if (e != hipSuccess) {
         delete pl;
@@ -516,13 +631,22 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
     }
     char *base = (char *)pl->d_mem;
     pl->d_xy = (double2 *)base;
-    pl->d_v = (double *)(base + al(bytes_xy));
-    pl->d_start = (int *)(base + al(bytes_xy) + al(bytes_v));
-    pl->d_perm = (int *)(base + al(bytes_xy) + al(bytes_v) + al(bytes_start));
+    base += al(bytes_xy);
+    pl->d_v = (double *)base;
+    base += al(bytes_v);
+    pl->d_start = (int *)base;
+    base += al(bytes_start);
+    pl->d_count = (int *)base;
+    base += al(bytes_count);
+    pl->d_perm = (int *)base;
+    base += al(bytes_perm);
+    pl->d_bbox = (double4 *)base;
     e = hipMemcpyAsync(pl->d_xy, xy.data(), bytes_xy, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(pl->d_v, v.data(), bytes_v, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(pl->d_start, start.data(), bytes_start, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->d_count, count.data(), bytes_count, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(pl->d_perm, pl->h_perm.data(), bytes_perm, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->d_bbox, bbox.data(), bytes_bbox, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         hipFree(pl->d_mem);
@@ -551,9 +675,10 @@ int pargo_idw_set_values(pargo_ctx *ctx, pargo_idw_plan *plan, const double *val
     if (!plan || !vals || plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "pargo_idw_set_values: bad arguments");
     int rc = pargo::bind(ctx);
     if (rc) return rc;
-    std::vector<double> v(plan->n);
-    for (int q = 0; q < plan->n; q++) v[q] = vals[plan->h_perm[q]];
-    PARGO_HIP(ctx, hipMemcpyAsync(plan->d_v, v.data(), sizeof(double) * plan->n, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> v((size_t)plan->slots, 0.0);
+    for (int q = 0; q < plan->slots; q++)
+        if (plan->h_perm[q] >= 0) v[q] = vals[plan->h_perm[q]];
+    PARGO_HIP(ctx, hipMemcpyAsync(plan->d_v, v.data(), sizeof(double) * plan->slots, hipMemcpyHostToDevice, ctx->stream));
     PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PARGO_OK;
 }
@@ -614,7 +739,7 @@ int pargo_idw_load(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int granul
     rc = pargo::ensure_scratch(ctx, sizeof(unsigned));
     if (rc) return rc;
     pargo::IdwLoadParams p{};
-    p.start = plan->d_start;
+    p.count = plan->d_count;
     p.mask = mask;
     p.out = load_out;
     p.nonterm = (unsigned *)ctx->scratch;
@@ -654,17 +779,17 @@ int pargo_idw_adjoint(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int pow
     if (!grad_out || !grad_vals) return fail(ctx, PARGO_E_INVALID, "pargo_idw_adjoint: null gradient pointer");
     rc = pargo::bind(ctx);
     if (rc) return rc;
-    rc = pargo::ensure_scratch(ctx, sizeof(double) * (size_t)plan->n);
+    rc = pargo::ensure_scratch(ctx, sizeof(double) * (size_t)plan->slots);
     if (rc) return rc;
     p.grad_out = grad_out;
     p.grad_acc = (double *)ctx->scratch;
-    PARGO_HIP(ctx, hipMemsetAsync(p.grad_acc, 0, sizeof(double) * (size_t)plan->n, ctx->stream));
+    PARGO_HIP(ctx, hipMemsetAsync(p.grad_acc, 0, sizeof(double) * (size_t)plan->slots, ctx->stream));
     rc = pargo::launch_idw<true>(p, ctx->stream);
     if (rc) return fail(ctx, rc, "unsupported searchPointNum %d", k);
     rc = pargo::check_launch(ctx, "pargo_idw_adjoint");
     if (rc) return rc;
-    hipLaunchKernelGGL(pargo::idw_unbucket_kernel, dim3((plan->n + 255) / 256), dim3(256), 0, ctx->stream,
-                       (const double *)p.grad_acc, (const int *)plan->d_perm, plan->n, grad_vals);
+    hipLaunchKernelGGL(pargo::idw_unbucket_kernel, dim3((plan->slots + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const double *)p.grad_acc, (const int *)plan->d_perm, plan->slots, grad_vals);
     return pargo::check_launch(ctx, "pargo_idw_adjoint (unbucket)");
 }
 
