@@ -145,7 +145,7 @@ __device__ __forceinline__ void load8_uniform(const double2 *src, double2 c[8])
     const unsigned long long base = ((unsigned long long)hi << 32) | lo;
     u32x16 a, b;
     asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx16 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
-                 : "=s"(a), "=s"(b)
+                 : "=&s"(a), "=&s"(b)  // early clobber: the first load must not land on the address
                  : "s"(base)
                  : "memory");
 #pragma unroll
