@@ -712,10 +712,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // band, row strip) items from a device queue until it is empty.  Items are
 // numbered tile by tile (first[t] = first item of tile t; tiles are in
 // decreasing size so that the tail of the queue holds the small items).
+// Batch tiles carry no NoData mask, so the mask code is compiled out.
 template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
-__global__ __launch_bounds__(256) void focal_batch_kernel(const FocalParams *__restrict__ tiles,
-                                                          const unsigned *__restrict__ first, int ntiles,
-                                                          unsigned *__restrict__ queue, unsigned total)
+__device__ __forceinline__ void focal_batch_body(const FocalParams *__restrict__ tiles,
+                                                 const unsigned *__restrict__ first, int ntiles,
+                                                 unsigned *__restrict__ queue, unsigned total)
 {
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -734,9 +735,26 @@ __global__ __launch_bounds__(256) void focal_batch_kernel(const FocalParams *__r
         const int wx = (int)(local % (unsigned)p.waves_x), strip = (int)(local / (unsigned)p.waves_x);
         const int rbeg = strip * p.rows_per_strip;
         const int rend = min(rbeg + p.rows_per_strip, p.rows);
-        if (p.vec) focal_strip<CPT, true, DO_SLOPE, DO_D8, HORN>(p, wx * 64 * CPT, lane, rbeg, rend);
-        else focal_strip<CPT, false, DO_SLOPE, DO_D8, HORN>(p, wx * 64 * CPT, lane, rbeg, rend);
+        if (p.vec) focal_strip<CPT, true, DO_SLOPE, DO_D8, HORN, false>(p, wx * 64 * CPT, lane, rbeg, rend);
+        else focal_strip<CPT, false, DO_SLOPE, DO_D8, HORN, false>(p, wx * 64 * CPT, lane, rbeg, rend);
     }
+}
+
+template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+__global__ __launch_bounds__(256) void focal_batch_kernel(const FocalParams *__restrict__ tiles,
+                                                          const unsigned *__restrict__ first, int ntiles,
+                                                          unsigned *__restrict__ queue, unsigned total)
+{
+    focal_batch_body<CPT, DO_SLOPE, DO_D8, HORN>(tiles, first, ntiles, queue, total);
+}
+
+// The fused Horn slope + D8 batch at 4 columns per lane runs at four waves
+// per SIMD, like the single-raster fused kernel.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void focal_batch_kernel_fused(
+    const FocalParams *__restrict__ tiles, const unsigned *__restrict__ first, int ntiles,
+    unsigned *__restrict__ queue, unsigned total)
+{
+    focal_batch_body<4, true, true, true>(tiles, first, ntiles, queue, total);
 }
 
 // Zeroes the batch queue head (a kernel rather than a memset, so that the
@@ -1017,10 +1035,12 @@ template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
 static int batch_blocks(pargo_ctx *ctx)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>, 256, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-        per_cu = 2;
+    hipError_t e;
+    if constexpr (CPT == 4 && DO_SLOPE && DO_D8 && HORN)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel_fused, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>, 256, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 2;
     const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
     return per_cu * cus;
 }
@@ -1028,8 +1048,12 @@ static int batch_blocks(pargo_ctx *ctx)
 template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
 static void launch_batch_t(const BatchPlan &b, hipStream_t s)
 {
-    hipLaunchKernelGGL((focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>), dim3(b.blocks), dim3(256), 0, s,
-                       (const FocalParams *)b.d_tiles, (const unsigned *)b.d_first, b.ntiles, b.d_queue, b.total);
+    if constexpr (CPT == 4 && DO_SLOPE && DO_D8 && HORN)
+        hipLaunchKernelGGL(focal_batch_kernel_fused, dim3(b.blocks), dim3(256), 0, s, (const FocalParams *)b.d_tiles,
+                           (const unsigned *)b.d_first, b.ntiles, b.d_queue, b.total);
+    else
+        hipLaunchKernelGGL((focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>), dim3(b.blocks), dim3(256), 0, s,
+                           (const FocalParams *)b.d_tiles, (const unsigned *)b.d_first, b.ntiles, b.d_queue, b.total);
 }
 
 template <int CPT>
