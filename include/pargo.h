@@ -156,7 +156,7 @@ int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, in
                          int mode);
 
 /* Times the persistent launch for each candidate shape (columns per lane 2
- * or 4, rows per strip 16, 32 or 64) on this batch, keeps the fastest for
+ * or 4, rows per strip 16, 32, 64 or 128) on this batch, keeps the fastest for
  * later batches on ctx and reports it (outputs may be NULL). */
 int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
                          int *cols_per_lane, int *rows_per_strip);

@@ -719,19 +719,27 @@ __device__ __forceinline__ void focal_batch_body(const FocalParams *__restrict__
                                                  unsigned *__restrict__ queue, unsigned total)
 {
     const int lane = threadIdx.x & 63;
+    // the tile of the previous item: a wave's next item usually falls in the
+    // same (large) tile, and then needs no search and no parameter reload
+    unsigned f0 = 1, f1 = 0;
+    FocalParams p;
     for (;;) {
         unsigned item = 0;
         if (lane == 0) item = atomicAdd(queue, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= total) break;
-        int lo = 0, hi = ntiles - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (first[mid] <= item) lo = mid;
-            else hi = mid - 1;
+        if (!(item >= f0 && item < f1)) {
+            int lo = 0, hi = ntiles - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (first[mid] <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            f0 = first[lo];
+            f1 = lo + 1 < ntiles ? first[lo + 1] : total;
+            p = tiles[lo];
         }
-        const FocalParams p = tiles[lo];
-        const unsigned local = item - first[lo];
+        const unsigned local = item - f0;
         const int wx = (int)(local % (unsigned)p.waves_x), strip = (int)(local / (unsigned)p.waves_x);
         const int rbeg = strip * p.rows_per_strip;
         const int rend = min(rbeg + p.rows_per_strip, p.rows);
@@ -1260,7 +1268,7 @@ int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, in
     if (!ctx) return PARGO_E_INVALID;
     int rc = pargo::bind(ctx);
     if (rc) return rc;
-    const int cpts[2] = {2, 4}, rps[3] = {16, 32, 64};
+    const int cpts[2] = {2, 4}, rps[4] = {16, 32, 64, 128};
     float best = INFINITY;
     int bc = ctx->batch_cpt, br = ctx->batch_rows;
     hipEvent_t e0, e1;
