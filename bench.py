@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--only", choices=("adjoint", "idw", "batch", "sca", "fill"), default=None,
+                    help="run the headline and this one secondary workload only (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy used to calibrate counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -389,20 +391,28 @@ def main():
     if not a.no_secondary:
         sec = {}
         t0 = time.time()
-        sec["adjoint"] = secondary_adjoint(torch, _lib, focal, synth)
-        log("adjoint done (%.1fs)" % (time.time() - t0))
-        sec["idw"] = secondary_idw(torch, synth, IDWOperator, not a.no_cpu_baseline)
-        log("idw done (%.1fs)" % (time.time() - t0))
-        sec["batch"] = secondary_batch(torch, batch)
-        log("batch done (%.1fs)" % (time.time() - t0))
-        zh = synth.workload_dem("slope_d8")
-        sec["sca"] = secondary_sca(torch, synth, focal, hydro, zh, not a.no_cpu_baseline)
-        log("sca done (%.1fs)" % (time.time() - t0))
-        sec["fill"] = secondary_fill(torch, synth, hydro, zh, not a.no_cpu_baseline)
-        log("fill done (%.1fs)" % (time.time() - t0))
-        del zh
+        want = (lambda name: a.only in (None, name))
+        if want("adjoint"):
+            sec["adjoint"] = secondary_adjoint(torch, _lib, focal, synth)
+            log("adjoint done (%.1fs)" % (time.time() - t0))
+        if want("idw"):
+            sec["idw"] = secondary_idw(torch, synth, IDWOperator, not a.no_cpu_baseline)
+            log("idw done (%.1fs)" % (time.time() - t0))
+        if want("batch"):
+            sec["batch"] = secondary_batch(torch, batch)
+            log("batch done (%.1fs)" % (time.time() - t0))
+        if want("sca") or want("fill"):
+            zh = synth.workload_dem("slope_d8")
+            if want("sca"):
+                sec["sca"] = secondary_sca(torch, synth, focal, hydro, zh, not a.no_cpu_baseline)
+                log("sca done (%.1fs)" % (time.time() - t0))
+            if want("fill"):
+                sec["fill"] = secondary_fill(torch, synth, hydro, zh, not a.no_cpu_baseline)
+                log("fill done (%.1fs)" % (time.time() - t0))
+            del zh
         out["secondary"] = sec
-        out["adjoint_fwd_ratio"] = sec["adjoint"]["adjoint_fwd_ratio"]
+        if "adjoint" in sec:
+            out["adjoint_fwd_ratio"] = sec["adjoint"]["adjoint_fwd_ratio"]
     print(json.dumps(out), flush=True)
 
 
