@@ -392,7 +392,8 @@ __device__ __forceinline__ float slope_unit(float s, int unit)
 // rlast (the strip's last input row) once r+4 passes it, so that a strip
 // fetches no row of the next one (a repeated row is a cache hit).
 // `special` is set when a window of this lane holds a value the fast path
-// does not handle.
+// does not handle.  MASK false compiles the NoData mask output out (p.mask
+// is then ignored); MASK true writes it when p.mask is set.
 template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK>
 __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlast, int c0, int lane, unsigned colb,
                                            Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw,

@@ -23,10 +23,11 @@
 // walks the ring sequence of one centre block at a time (once in the common
 // case, twice or four times for a tile on a block edge or corner), every
 // lane with that centre taking part: candidate addresses are wave-uniform,
-// so candidates arrive through scalar loads, four at a time, and each lane
-// only does its distance test.  A lane whose own search has ended (ring
-// stop, buffer stop) stays in the walk with a threshold no distance passes.
-// Lanes keep their K nearest (distance, bucket slot) pairs in registers.
+// so candidates arrive through scalar loads, eight per wait (a bucket's
+// last four alone), and each lane only does its distance test.  A lane
+// whose own search has ended (ring stop, buffer stop) stays in the walk
+// with a threshold no distance passes.  Each lane's K nearest (distance,
+// bucket slot) pairs live in LDS.
 //
 // Exactness: squared distances are formed as RN(RN(dx*dx) + RN(dy*dy)) from
 // dx = RN(xs - cx), the operands of the reference's sqrt(pow(dx,2) +
