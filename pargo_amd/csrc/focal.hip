@@ -895,10 +895,12 @@ static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, b
 static LaunchCfg default_cfg(int rows)
 {
     LaunchCfg c;
-    // 4 columns per lane and 32-row strips measured best for the fused
-    // kernel on 16384^2 (MI355X): 3 waves per SIMD, fewer halo drops per cell.
+    // 4 columns per lane and 16-row strips measured best for the fused
+    // kernel on 16384^2 (MI355X, 4 waves per SIMD): 0.495 ms against 0.500
+    // at 32 rows, 0.504 at 12, 0.52-0.58 at 48-128 rows (fewer, longer work
+    // items leave a ragged tail); the extra halo rows cost less than that.
     c.cpt = 4;
-    c.rows_per_strip = 32;
+    c.rows_per_strip = 16;
     if (const char *e = getenv("PARGO_FOCAL_CPT")) c.cpt = atoi(e) == 2 ? 2 : 4;
     if (const char *e = getenv("PARGO_FOCAL_ROWS")) {
         int v = atoi(e);
