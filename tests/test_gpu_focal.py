@@ -28,8 +28,13 @@ def upload(z, pitch_pad=0, col_offset=0):
 def gpu_run(z, cs, nodata, method="FD", dirtype=256, unit="tan", pitch_pad=0, col_offset=0):
     t = upload(z, pitch_pad, col_offset)
     s, c, m = focal.slope_d8(t, cs, nodata, method, unit, dirtype, want_mask=True)
+    # the call without a mask runs a kernel of its own (the mask-free fused
+    # pass for Horn slope): same slope and codes, bit for bit
+    s0, c0, _ = focal.slope_d8(t, cs, nodata, method, unit, dirtype)
     import torch
     torch.cuda.synchronize()
+    assert torch.equal(torch.nan_to_num(s0, nan=-1.0), torch.nan_to_num(s, nan=-1.0))
+    assert torch.equal(c0, c)
     return s.cpu().numpy(), c.cpu().numpy(), focal.unpack_mask(m, z.shape[1]).cpu().numpy()
 
 
