@@ -21,10 +21,17 @@
 //
 // Device order: 64 x 32 tiles.  A workgroup loads its tile and a one-cell
 // halo of W into LDS and sweeps the update over the tile (each wave down and
-// up its 8 rows) until nothing in it changes; then it writes the tile back and, for each changed edge,
-// schedules the neighbouring tiles for the next pass.  Passes repeat until
-// one changes nothing.  Tiles read each other's edges while they change;
-// that only feeds the monotone iteration earlier values, never wrong ones.
+// up its 8 rows) until nothing in it changes; then it writes the tile back
+// and, for each changed edge, schedules the neighbouring tiles for the next
+// pass.  Passes repeat until one changes nothing.  Tiles read each other's
+// edges while they change; that only feeds the monotone iteration earlier
+// values, never wrong ones.
+//
+// Only scheduled tiles run: a pass walks a device worklist with a fixed
+// grid, and a tile enters the next pass's list when it is first stamped for
+// it.  Pass 0's list holds the tiles with a NoData or border cell within
+// their one-cell halo: with every valid W at 10000, nothing else can change
+// (a cell moves only on a NoData neighbour or a neighbour below 10000).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -42,8 +49,10 @@ constexpr double kStart = 10000.0; // pitRemoveOperator.cpp:41
 struct FillParams {
     const float *__restrict__ dem;
     double *__restrict__ w;
-    int *__restrict__ stamp;         // per tile: the pass that must process it
+    int *__restrict__ stamp;         // per tile: the pass that must process it (-1: none yet)
     unsigned *__restrict__ changed;  // per pass of the current batch: tiles that changed
+    unsigned *__restrict__ count;    // [2]: entries of the worklists of even and odd passes
+    int *__restrict__ list;          // [2][tiles]: tiles of the pass (even, odd)
     long long pitch;
     int rows, cols, tiles_x, tiles_y;
     float nd_lo, nd_hi;  // float32 values with fabs((double)v - nodata) <= 1e-6
@@ -52,29 +61,53 @@ struct FillParams {
 
 __device__ __forceinline__ bool is_nd(const FillParams &p, float v) { return v >= p.nd_lo && v <= p.nd_hi; }
 
-// W before the first sweep (pitRemoveOperator.cpp:39-45 plus copyLayerInfo).
+// Puts tile t on the worklist of pass `pass` unless it is there already
+// (its stamp says so).
+__device__ __forceinline__ void schedule(const FillParams &p, int t, int pass)
+{
+    if (atomicExch(&p.stamp[t], pass) != pass) {
+        const unsigned k = atomicAdd(&p.count[pass & 1], 1u);
+        p.list[(long long)(pass & 1) * p.tiles_x * p.tiles_y + k] = t;
+    }
+}
+
+// W before the first sweep (pitRemoveOperator.cpp:39-45 plus copyLayerInfo),
+// and pass 0's worklist: every tile holding a NoData or border cell in its
+// one-cell halo.
 __global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
 {
     const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
     if (c >= p.cols) return;
     const long long i = (long long)r * p.pitch + c;
     const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
-    p.w[i] = (interior && !is_nd(p, p.dem[i])) ? kStart : p.nodata;
+    const bool valid = interior && !is_nd(p, p.dem[i]);
+    p.w[i] = valid ? kStart : p.nodata;
+    if (!valid) {
+        // the tiles whose cells or halo hold (r, c)
+        const int ty0 = max(r - 1, 0) / kTH, ty1 = min(r + 1, p.rows - 1) / kTH;
+        const int tx0 = max(c - 1, 0) / kTW, tx1 = min(c + 1, p.cols - 1) / kTW;
+        for (int ty = ty0; ty <= ty1; ty++)
+            for (int tx = tx0; tx <= tx1; tx++) schedule(p, ty * p.tiles_x + tx, 0);
+    }
 }
 
-__global__ void fill_stamp_init_kernel(int *stamp, int n, unsigned *changed, int npass)
+// Stamps to -1 and both worklists empty (before the first pass), or the
+// per-pass change counters of a batch to zero.
+__global__ void fill_stamp_init_kernel(int *stamp, int n, unsigned *changed, int npass, unsigned *count)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) stamp[i] = 0;
+    if (i < n) stamp[i] = -1;
     if (i < npass) changed[i] = 0;
+    if (n > 0 && i < 2) count[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, int batch_first)
+// Empties the worklist that pass `pass` + 1 fills (the one pass - 1 read).
+__global__ void fill_list_reset_kernel(unsigned *count, int pass) { count[(pass + 1) & 1] = 0; }
+
+// One pass over one tile (see the top of the file).
+__device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pass, int batch_first,
+                                          double (&sw)[kTH + 2][kTW + 2], int &edge_flags, int &run, int tid)
 {
-    const int tile = blockIdx.x;
-    __shared__ double sw[kTH + 2][kTW + 2];
-    __shared__ int edge_flags, run;
-    const int tid = threadIdx.x;
     // one read of the stamp for the whole workgroup: a neighbour may rewrite
     // it meanwhile (to pass + 1, which defers this tile by one pass)
     if (tid == 0) {
@@ -82,7 +115,7 @@ __global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, 
         run = p.stamp[tile] == pass;
     }
     __syncthreads();
-    if (!run) return;  // nothing around it changed since it settled
+    if (!run) return;  // moved on to the next pass meanwhile
     const int ty = tile / p.tiles_x, tx = tile - ty * p.tiles_x;
     const int r0 = ty * kTH, c0 = tx * kTW;
     for (int i = tid; i < (kTH + 2) * (kTW + 2); i += 256) {
@@ -172,12 +205,26 @@ __global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, 
                                   (dx == 1 && (f & 8));
                 const int nty = ty + dy, ntx = tx + dx;
                 if (need && nty >= 0 && nty < p.tiles_y && ntx >= 0 && ntx < p.tiles_x)
-                    p.stamp[nty * p.tiles_x + ntx] = next;
+                    schedule(p, nty * p.tiles_x + ntx, next);
             }
         }
     }
 }
 
+
+__global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, int batch_first)
+{
+    __shared__ double sw[kTH + 2][kTW + 2];
+    __shared__ int edge_flags, run;
+    const int tid = threadIdx.x;
+    const unsigned n = p.count[pass & 1];
+    const int *list = p.list + (long long)(pass & 1) * p.tiles_x * p.tiles_y;
+    for (unsigned e = blockIdx.x; e < n; e += gridDim.x) {
+        const int tile = list[e];
+        fill_tile(p, tile, pass, batch_first, sw, edge_flags, run, tid);
+        __syncthreads();  // the next tile reuses the LDS tile
+    }
+}
 // Smallest valid elevation (as ordered bits of a float) and a NaN count,
 // for the monotonicity check: grid-stride over the interior, one atomic per
 // workgroup.
@@ -256,14 +303,17 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
     const int ntiles = p.tiles_x * p.tiles_y;
     constexpr int kBatch = 8;      // passes between checks
     constexpr int kMaxPasses = 1 << 20;
-    // scratch: [0] min bits, [1] NaN count, then per-pass counters, then stamps
-    const size_t counters = 2 + (size_t)kBatch;
-    rc = pargo::ensure_scratch(ctx, sizeof(unsigned) * counters + sizeof(int) * (size_t)ntiles + 256);
+    // scratch: [0] min bits, [1] NaN count, per-pass counters, the two
+    // worklist counts, then stamps and the two worklists
+    const size_t counters = 2 + (size_t)kBatch + 2;
+    rc = pargo::ensure_scratch(ctx, sizeof(unsigned) * counters + sizeof(int) * 3 * (size_t)ntiles + 256);
     if (rc) return rc;
     unsigned *scr = static_cast<unsigned *>(ctx->scratch);
     unsigned *minbits = scr, *nans = scr + 1, *changed = scr + 2;
-    p.stamp = reinterpret_cast<int *>(scr + counters);
     p.changed = changed;
+    p.count = scr + 2 + kBatch;
+    p.stamp = reinterpret_cast<int *>(scr + counters);
+    p.list = p.stamp + ntiles;
 
     // monotonicity check: every valid W lies in [min(dem, 10000), 10000]
     unsigned host2[2] = {0xffffffffu, 0u};
@@ -289,14 +339,22 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
                         "result would depend on its sweep order",
                         g->nodata, lo);
     }
+    // stamps at -1 and empty worklists, then W and pass 0's worklist
+    hipLaunchKernelGGL(pargo::fill_stamp_init_kernel, dim3((std::max(ntiles, kBatch) + 255) / 256), dim3(256), 0,
+                       ctx->stream, p.stamp, ntiles, changed, kBatch, p.count);
     hipLaunchKernelGGL(pargo::fill_init_kernel, rowsgrid, dim3(256), 0, ctx->stream, p);
+    // a fixed grid walks each pass's worklist: 8 workgroups per CU
+    const int pass_blocks = std::min(ntiles, cus * 8);
     int pass = 0;
     for (;;) {
-        // stamps start at 0 (every tile runs in pass 0); counters per batch
-        hipLaunchKernelGGL(pargo::fill_stamp_init_kernel, dim3(pass == 0 ? (std::max(ntiles, kBatch) + 255) / 256 : 1),
-                           dim3(256), 0, ctx->stream, p.stamp, pass == 0 ? ntiles : 0, changed, kBatch);
-        for (int k = 0; k < kBatch; k++)
-            hipLaunchKernelGGL(pargo::fill_pass_kernel, dim3(ntiles), dim3(256), 0, ctx->stream, p, pass + k, pass);
+        // change counters per batch
+        if (pass > 0)
+            hipLaunchKernelGGL(pargo::fill_stamp_init_kernel, dim3(1), dim3(256), 0, ctx->stream, p.stamp, 0, changed,
+                               kBatch, p.count);
+        for (int k = 0; k < kBatch; k++) {
+            hipLaunchKernelGGL(pargo::fill_list_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, p.count, pass + k);
+            hipLaunchKernelGGL(pargo::fill_pass_kernel, dim3(pass_blocks), dim3(256), 0, ctx->stream, p, pass + k, pass);
+        }
         unsigned h[kBatch];
         PARGO_HIP(ctx, hipMemcpyAsync(h, changed, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
         PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
