@@ -56,7 +56,7 @@ namespace pargo {
 namespace {
 
 constexpr unsigned kNone = 15;
-constexpr int kTileC = 64, kTileR = 8;  // cells per workgroup: 64 x 8
+constexpr int kTileC = 64, kTileR = 32;  // cells per workgroup: 64 x 32
 constexpr uint64_t kAccMask = (1ull << 40) - 1;
 
 __device__ __forceinline__ unsigned flow_of(unsigned b) { return (b <= 8u && b != 4u) ? b : kNone; }
@@ -84,7 +84,7 @@ __device__ __forceinline__ unsigned load_class(const ScaParams &p, int r, int c)
     return p.flow[(long long)r * p.pitch + c];
 }
 
-// Static words of a 64 x 8 tile.  Classes of the tile plus a 2-cell halo
+// Static words of a 64 x 32 tile.  Classes of the tile plus a 2-cell halo
 // go to LDS, then the inflow mask of the tile plus a 1-cell halo (bit j set
 // when the neighbour at window position j < 4 ? j : j + 1 drains into the
 // cell and is not excluded; a cell needs to know which of its inflows are
