@@ -43,6 +43,7 @@ namespace pargo {
 namespace {
 
 constexpr int kTW = 64, kTH = 32;  // tile: 64 columns x 32 rows, 8 cells per thread
+static_assert(kTW == 64, "fill_init_kernel takes a wave's 64 columns for one tile column");
 constexpr double kGap = 0.0005;    // pitRemoveOperator.cpp:36
 constexpr double kStart = 10000.0; // pitRemoveOperator.cpp:41
 
@@ -82,13 +83,25 @@ __global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
     const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
     const bool valid = interior && !is_nd(p, p.dem[i]);
     p.w[i] = valid ? kStart : p.nodata;
-    if (!valid) {
-        // the tiles whose cells or halo hold (r, c)
-        const int ty0 = max(r - 1, 0) / kTH, ty1 = min(r + 1, p.rows - 1) / kTH;
-        const int tx0 = max(c - 1, 0) / kTW, tx1 = min(c + 1, p.cols - 1) / kTW;
-        for (int ty = ty0; ty <= ty1; ty++)
-            for (int tx = tx0; tx <= tx1; tx++) schedule(p, ty * p.tiles_x + tx, 0);
-    }
+    // The tiles whose cells or halo hold an excluded cell of this wave.  A
+    // wave is 64 columns of one row starting at a multiple of kTW, so they
+    // are its own tile column w, w - 1 when its first cell is excluded and
+    // w + 1 when its last one is, in the tile rows of r - 1 .. r + 1.  Lane 0
+    // (the first cell, active whenever any lane is) schedules them, once per
+    // wave rather than once per cell: same-address atomics serialise.
+    const unsigned long long nd = __builtin_amdgcn_ballot_w64(!valid);
+    if (nd == 0 || (threadIdx.x & 63) != 0) return;
+    const int w = c / kTW;
+    const int tx0 = (nd & 1ull) && w > 0 ? w - 1 : w;
+    const int tx1 = (nd >> 63) && (w + 1) * kTW <= p.cols - 1 ? w + 1 : w;
+    const int ty0 = max(r - 1, 0) / kTH, ty1 = min(r + 1, p.rows - 1) / kTH;
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            const int t = ty * p.tiles_x + tx;
+            // a plain read skips the atomic once the stamp is set: stamps only
+            // go from -1 to 0 here, so a stale -1 costs an atomic, nothing more
+            if (p.stamp[t] != 0) schedule(p, t, 0);
+        }
 }
 
 // Stamps to -1 and both worklists empty (before the first pass), or the
