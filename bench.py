@@ -268,8 +268,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy used to calibrate counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path; "
+              "a larger DEM would be split into row bands with a one-row halo, one band per GPU",
+              file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         copy_probe()
