@@ -67,6 +67,20 @@ def test_upstream_propagation_across_tiles():
     assert np.array_equal(got, ref) and passes > 2
 
 
+def test_nodata_at_wave_and_tile_edges():
+    # NoData cells in the first and last column of a 64-column wave and on
+    # the rows either side of a 32-row tile edge, away from the border tiles:
+    # the first pass must already hold the tiles whose halo they sit in
+    rng = np.random.default_rng(530)
+    z = np.round(100 + rng.random((200, 300)) * 30, 2).astype(np.float32)
+    for r, c in [(80, 128), (150, 191), (95, 100), (96, 64), (127, 255), (63, 63)]:
+        z[r, c] = ND
+    ref, _, ok = pyoracle.pit_remove(z, ND)
+    assert ok
+    got, _ = gpu_fill(z)
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5].tolist()
+
+
 def test_nodata_inside_filled_range_is_refused():
     import torch
     z = torch.full((16, 16), 50.0, dtype=torch.float32, device="cuda")
