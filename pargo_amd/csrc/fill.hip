@@ -43,6 +43,7 @@ namespace pargo {
 namespace {
 
 constexpr int kTW = 64, kTH = 32;  // tile: 64 columns x 32 rows, 8 cells per thread
+constexpr int kRW = kTH / 4;       // rows per wave (4 waves of kTW lanes)
 static_assert(kTW == 64, "fill_init_kernel takes a wave's 64 columns for one tile column");
 constexpr double kGap = 0.0005;    // pitRemoveOperator.cpp:36
 constexpr double kStart = 10000.0; // pitRemoveOperator.cpp:41
@@ -138,11 +139,11 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
     }
     // own cells: wave w owns tile rows 8w .. 8w+7, lane = column
     const int lc = tid % kTW + 1;
-    const int rbase = (tid / kTW) * 8;  // first tile row of this wave
-    double d[8];
+    const int rbase = (tid / kTW) * kRW;  // first tile row of this wave
+    double d[kRW];
     unsigned live = 0;  // bit k: tile row rbase + k holds a valid interior cell
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < kRW; k++) {
         const int r = r0 + rbase + k, c = c0 + lc - 1;
         d[k] = 0.0;
         if (r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2) {
@@ -185,12 +186,12 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
     for (;;) {
         bool any = false;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < kRW; k++) {
             any |= update(k);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
 #pragma unroll
-        for (int k = 7; k >= 0; k--) {
+        for (int k = kRW - 1; k >= 0; k--) {
             any |= update(k);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
@@ -198,7 +199,7 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
     }
     unsigned flags = 0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < kRW; k++) {
         if (!((touched >> k) & 1u)) continue;
         const int tr = rbase + k, tc = lc - 1;
         const int r = r0 + tr, c = c0 + tc;
