@@ -137,13 +137,15 @@ __device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, in
 }
 
 // Cross-lane moves by DPP wavefront shifts (lane i reads lane i-1 / i+1).
-__device__ __forceinline__ float from_lower_lane(float v)
+// The lane without a source (0, resp. 63) is not written by the move and
+// returns `edge`, the old value of the destination.
+__device__ __forceinline__ float from_lower_lane(float v, float edge)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float from_upper_lane(float v)
+__device__ __forceinline__ float from_upper_lane(float v, float edge)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
 }
 
 // OR of v over each aligned group of N lanes (N = 4, 8 or 16), every lane
@@ -262,10 +264,9 @@ template <int CPT, bool KEEP_Z>
 __device__ __forceinline__ void finish_row(const FocalParams &p, const RawRow<CPT> &raw, int lane, Slot<CPT> &o)
 {
     float f[CPT + 2];
-    const float left = from_lower_lane(raw.v[CPT - 1]);
-    const float right = from_upper_lane(raw.v[0]);
-    f[0] = (lane == 0) ? raw.hl : left;
-    f[CPT + 1] = (lane == 63) ? raw.hl : right;
+    // lanes 0 and 63 have no source lane and keep the halo value they loaded
+    f[0] = from_lower_lane(raw.v[CPT - 1], raw.hl);
+    f[CPT + 1] = from_upper_lane(raw.v[0], raw.hl);
 #pragma unroll
     for (int k = 0; k < CPT; k++) f[k + 1] = raw.v[k];
     float mn = f[0], mx = f[0];
@@ -572,7 +573,9 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlas
         uint8_t *crow = p.code + rbase;
         if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
-                *reinterpret_cast<unsigned *>(crow + uc0) = cd[0] | (cd[1] << 8) | (cd[2] << 16) | (cd[3] << 24);
+                *reinterpret_cast<unsigned *>(crow + uc0) = __builtin_amdgcn_perm(
+                    __builtin_amdgcn_perm(cd[3], cd[2], 0x0c0c0400u), __builtin_amdgcn_perm(cd[1], cd[0], 0x0c0c0400u),
+                    0x05040100u);  // byte 0 of each code, three byte permutes
             } else if constexpr (CPT == 2) {
                 *reinterpret_cast<unsigned short *>(crow + uc0) = (unsigned short)(cd[0] | (cd[1] << 8));
             } else {
