@@ -145,8 +145,10 @@ template <bool VEC>
 __global__ __launch_bounds__(256) void slope_adjoint_kernel(AdjParams p)
 {
     const int lane = threadIdx.x & 63;
-    const int c0 = (blockIdx.x * blockDim.x + threadIdx.x) * ACPT;
-    const int rbeg = blockIdx.y * p.rows_per_strip;
+    unsigned cb, strip;
+    xcd_strip_order(cb, strip);
+    const int c0 = (int)(cb * blockDim.x + threadIdx.x) * ACPT;
+    const int rbeg = (int)strip * p.rows_per_strip;
     const int rend = min(rbeg + p.rows_per_strip, p.rows);
     const int wave_c0 = __builtin_amdgcn_readfirstlane(c0);
     const bool full = wave_c0 + 64 * ACPT <= p.cols;

@@ -680,8 +680,16 @@ __device__ __forceinline__ void focal_raster(const FocalParams &p)
 {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wave_c0 = (blockIdx.x * 4 + wave) * 64 * CPT;
-    const int rbeg = blockIdx.y * p.rows_per_strip;
+    // The kernels that may write a NoData mask take the XCD-contiguous strip
+    // order (5 % less read traffic; 1.3 % faster for the fused pass with the
+    // mask at 16384^2, 9 % for slope alone at 8192^2).  The fused kernel
+    // without the mask keeps the launch order, which spreads the strips in
+    // flight over the whole raster: renumbered, it read as little but ran
+    // 5 % slower.
+    unsigned cb = blockIdx.x, strip = blockIdx.y;
+    if (MASK) xcd_strip_order(cb, strip);
+    const int wave_c0 = (int)(cb * 4 + wave) * 64 * CPT;
+    const int rbeg = (int)strip * p.rows_per_strip;
     const int rend = min(rbeg + p.rows_per_strip, p.rows);
     focal_strip<CPT, VEC, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
 }

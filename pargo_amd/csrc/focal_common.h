@@ -49,4 +49,18 @@ struct D8Lut8 {
     uint8_t v[256];
 };
 
+// Workgroups go to the 8 XCDs in turn by linear id.  Renumbers the
+// workgroup of a (column block, row strip) grid so that each XCD takes a
+// contiguous run of strips, column blocks first: neighbouring strips then
+// share their halo rows in that XCD's L2 instead of reading them twice from
+// HBM.  A bijection of the grid onto itself.
+__device__ __forceinline__ void xcd_strip_order(unsigned &cb, unsigned &strip)
+{
+    const unsigned gx = gridDim.x, n = gx * gridDim.y, i = blockIdx.y * gx + blockIdx.x;
+    const unsigned per = n / 8, rem = n % 8, x = i % 8;
+    const unsigned w = x * per + (x < rem ? x : rem) + i / 8;
+    cb = w % gx;
+    strip = w / gx;
+}
+
 }  // namespace pargo
