@@ -98,7 +98,11 @@ const char *pargo_last_error(const pargo_ctx *ctx);
 int pargo_abi_version(void);
 /* Runs subsequent work on `hip_stream` (a hipStream_t; NULL is HIP's
  * default stream) instead of the context's own stream.
- * pargo_set_stream(ctx, pargo_own_stream(ctx)) switches back. */
+ * pargo_set_stream(ctx, pargo_own_stream(ctx)) switches back.
+ * One stream at a time: a context's device state (fixup queue, scratch,
+ * batch queue) is shared by everything queued through it, so a switch makes
+ * the new stream wait (hipStreamWaitEvent) for the work already queued on
+ * the old one.  Work for two streams that must overlap needs two contexts. */
 int pargo_set_stream(pargo_ctx *ctx, void *hip_stream);
 void *pargo_own_stream(const pargo_ctx *ctx);
 int pargo_sync(pargo_ctx *ctx);
@@ -151,7 +155,11 @@ typedef struct {
  * gpro/deComposition.h:148-182) and balances bands by estimated load
  * (ComputeLayer, gpro/computeLayer.h:378-443); here (tile, column band, row
  * strip) items of many rasters are handed to waves from one queue, largest
- * tiles first.  Results are identical to per-tile pargo_slope_d8 calls. */
+ * tiles first.  The three modes give identical bits.  They equal per-tile
+ * pargo_slope_d8 calls bit for bit, except that a batch holding a tile whose
+ * cell size is outside [2^-20, 2^40] takes the generic FP64 Horn path for
+ * every tile: then the other tiles' slopes are within 1e-5 of pargo_slope_d8
+ * (which takes the FP32-root path for them) and their D8 codes identical. */
 int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
                          int mode);
 
@@ -241,6 +249,12 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
 int pargo_idw_plan_free(pargo_ctx *ctx, pargo_idw_plan *plan);
 /* Replaces new sample values (host array, file order) without re-bucketing. */
 int pargo_idw_set_values(pargo_ctx *ctx, pargo_idw_plan *plan, const double *vals);
+/* The same from a device array (file order, n doubles): a gather into the
+ * plan's bucket order, queued on the context's stream with no host
+ * synchronisation, so a loop that updates the values and reruns IDW never
+ * waits for the host.  Replaces the host-side value reload between two
+ * Run() calls of IDWOperator (idwOperator.cpp:63-149 reads values once). */
+int pargo_idw_set_values_device(pargo_ctx *ctx, pargo_idw_plan *plan, const double *d_vals);
 
 /* IDW of every output cell: IDWOperator::Operator + searchNbrSamples
  * (idwOperator.cpp:332-491) with 1-rank semantics.  out is a device float

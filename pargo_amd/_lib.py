@@ -54,6 +54,7 @@ _SIGS = {
     "pargo_idw_prepare": (_c.c_int, [_P, _P, _P, _P, _c.c_int, _c.c_float, _c.c_double, _c.POINTER(_P)]),
     "pargo_idw_plan_free": (_c.c_int, [_P, _P]),
     "pargo_idw_set_values": (_c.c_int, [_P, _P, _P]),
+    "pargo_idw_set_values_device": (_c.c_int, [_P, _P, _P]),
     "pargo_idw_run": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int, _c.c_int,
                                  _P, _P]),
     "pargo_idw_load": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
@@ -147,6 +148,36 @@ def missing_symbols():
     """Entry points of include/pargo.h that the loaded library lacks."""
     L = lib()
     return [n for n in _SIGS if getattr(L, n, None) is None]
+
+
+def row_pitch(t, name="raster"):
+    """Row pitch in elements of a 2-D row-major tensor, refusing layouts the
+    kernels cannot address: rows must be contiguous (stride 1 along a row)
+    and must not overlap (stride(0) >= cols when there is more than one row;
+    a broadcast tensor such as x.expand(rows, cols) has stride(0) == 0).
+    Checked before any device work, so a bad layout never reaches a launch."""
+    if t.dim() != 2:
+        raise ValueError("%s must be 2-D, got %d dimensions" % (name, t.dim()))
+    rows, cols = int(t.shape[0]), int(t.shape[1])
+    if cols > 1 and t.stride(1) != 1:
+        raise ValueError("%s rows must be contiguous (stride(1) == 1), got strides %s" % (name, tuple(t.stride())))
+    if rows > 1 and t.stride(0) < cols:
+        raise ValueError("%s rows overlap: stride(0) = %d < %d columns (a broadcast or expanded tensor); "
+                         "pass a contiguous copy" % (name, t.stride(0), cols))
+    return int(t.stride(0)) if rows > 1 else cols
+
+
+def check_raster(t, dtype, name):
+    """A 2-D tensor of `dtype` with addressable rows (row_pitch) on a HIP
+    device; the layout is checked first so the error names it on any device."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor" % name)
+    if t.dtype != dtype:
+        raise TypeError("%s must be a %s tensor, got %s" % (name, dtype, t.dtype))
+    row_pitch(t, name)
+    if t.device.type != "cuda":
+        raise TypeError("%s must be on a HIP device" % name)
 
 
 def _ptr(t):

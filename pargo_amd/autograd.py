@@ -30,16 +30,26 @@ class SlopeFunction(torch.autograd.Function):
 
 
 def slope_adjoint(dem, grad_slope, cellsize, nodata, out=None):
+    focal._check_dem(dem)
+    grid = focal.grid_of(dem, cellsize, nodata)
     c = _lib.default_context(dem.device.index or 0)
     c.bind_stream()
-    g = grad_slope.contiguous() if grad_slope.stride(1) != 1 else grad_slope
-    if g.stride(0) != dem.stride(0):
-        g2 = torch.empty_strided(dem.shape, dem.stride(), dtype=torch.float32, device=dem.device)
-        g2.copy_(g)
-        g = g2
+    if tuple(grad_slope.shape) != tuple(dem.shape):
+        raise ValueError("grad_slope has shape %s, the DEM %s" % (tuple(grad_slope.shape), tuple(dem.shape)))
+    pitch = grid.pitch
+
+    def same_layout(t):
+        return (t.dtype == torch.float32 and t.device == dem.device and t.dim() == 2
+                and (t.shape[1] <= 1 or t.stride(1) == 1) and (t.shape[0] <= 1 or t.stride(0) == pitch))
+
+    g = grad_slope
+    if not same_layout(g):
+        g = focal._alloc_like(dem, torch.float32)
+        g.copy_(grad_slope)
     if out is None:
-        out = torch.empty_strided(dem.shape, dem.stride(), dtype=torch.float32, device=dem.device)
-    grid = focal.grid_of(dem, cellsize, nodata)
+        out = focal._alloc_like(dem, torch.float32)
+    elif tuple(out.shape) != tuple(dem.shape) or not same_layout(out):
+        raise ValueError("out must be a float32 tensor of the DEM's shape and row pitch")
     c.check(_lib.lib().pargo_slope_adjoint(c.h, ctypes.byref(grid), _lib._ptr(dem), _lib._ptr(g), _lib._ptr(out)))
     return out
 
@@ -51,7 +61,8 @@ def slope(dem, cellsize=30.0, nodata=-9999.0):
 class IDWFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, op):
-        op.set_values(values.detach().cpu().double().numpy())
+        # device values stay on the device: no host round trip per step
+        op.set_values(values.detach().to(torch.float64).contiguous())
         out = op.run()
         ctx.op = op
         return out

@@ -24,8 +24,11 @@ class PargoTile(ctypes.Structure):
 
 def _tile_array(dems, slopes, codes, cellsize, nodata):
     arr = (PargoTile * len(dems))()
+    sizes = list(cellsize) if isinstance(cellsize, (list, tuple)) else [cellsize] * len(dems)
+    if len(sizes) != len(dems):
+        raise ValueError("one cell size per tile, or one for all")
     for i, d in enumerate(dems):
-        arr[i].grid = focal.grid_of(d, cellsize, nodata)
+        arr[i].grid = focal.grid_of(d, sizes[i], nodata)
         arr[i].dem = d.data_ptr()
         arr[i].slope_out = slopes[i].data_ptr() if slopes is not None else None
         arr[i].code_out = codes[i].data_ptr() if codes is not None else None
@@ -37,15 +40,20 @@ def _setup_lib():
 
 
 class Batch:
-    """A fixed set of device rasters and their outputs."""
+    """A fixed set of device rasters and their outputs.  cellsize is one
+    value for every tile or a list with one per tile."""
 
     def __init__(self, dems, cellsize=30.0, nodata=-9999.0, want_slope=True, want_d8=True, method="FD", unit="tan",
                  dirtype=256, ctx=None):
         import torch
+        for i, d in enumerate(dems):
+            _lib.check_raster(d, torch.float32, "dems[%d]" % i)
         self.dems = dems
         self.ctx = ctx or _lib.default_context(dems[0].device.index or 0)
-        self.slopes = [torch.empty_like(d) for d in dems] if want_slope else None
-        self.codes = [torch.empty(d.shape, dtype=torch.uint8, device=d.device) for d in dems] if want_d8 else None
+        # outputs share each DEM's row pitch (the kernels address all three
+        # rasters of a tile with one pitch)
+        self.slopes = [focal._alloc_like(d, torch.float32) for d in dems] if want_slope else None
+        self.codes = [focal._alloc_like(d, torch.uint8) for d in dems] if want_d8 else None
         self.tiles = _tile_array(dems, self.slopes, self.codes, cellsize, nodata)
         self.method = method if isinstance(method, int) else focal.get_slope_algorithm(method)
         self.unit = focal._unit(unit)

@@ -4,6 +4,7 @@ Usage: python -m pargo_amd.build [--jobs N] [--no-oracle]
 
 Outputs (git-ignored, shipped in-tree):
   pargo_amd/_build/libpargo.so       the product library behind include/pargo.h
+  pargo_amd/_build/libpargo_ceiling.so  streaming kernels for bench.py's memory ceiling
   pargo_amd/_build/bin/{slope,flowdird8,idw}   drop-in command lines
   oracle/_build/liboracle.so         CPU oracle (test infrastructure only)
 """
@@ -65,7 +66,18 @@ def build_library(jobs=8):
     tmp = os.path.join(BUILD, "libpargo.so.tmp")
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lgomp"])
     os.replace(tmp, os.path.join(BUILD, "libpargo.so"))
+    build_ceiling()
     return os.path.join(BUILD, "libpargo.so")
+
+
+def build_ceiling():
+    """libpargo_ceiling.so: the streaming kernels bench.py measures the
+    memory system with (a measurement helper, not part of the product)."""
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = os.path.join(BUILD, "libpargo_ceiling.so.tmp")
+    _run([HIPCC] + HIP_FLAGS + ["-shared", os.path.join(CSRC, "ceiling.hip"), "-o", tmp])
+    os.replace(tmp, os.path.join(BUILD, "libpargo_ceiling.so"))
+    return os.path.join(BUILD, "libpargo_ceiling.so")
 
 
 GPRO = os.path.join(PKG, "gpro")

@@ -165,6 +165,7 @@ int pargo_destroy(pargo_ctx *ctx)
     if (ctx->scratch) hipFree(ctx->scratch);
     pargo::free_batch_plan(ctx);
     if (ctx->fix_mem) hipFree(ctx->fix_mem);
+    if (ctx->switch_event) hipEventDestroy(ctx->switch_event);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return PARGO_OK;
@@ -179,7 +180,16 @@ const char *pargo_last_error(const pargo_ctx *ctx)
 int pargo_set_stream(pargo_ctx *ctx, void *hip_stream)
 {
     if (!ctx) return PARGO_E_INVALID;
-    ctx->stream = (hipStream_t)hip_stream;  // NULL is HIP's default stream
+    const hipStream_t ns = (hipStream_t)hip_stream;  // NULL is HIP's default stream
+    if (ns == ctx->stream) return PARGO_OK;
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    // Work already queued on the old stream uses the context's device state
+    // (fixup queue, scratch, batch queue): order the new stream after it.
+    if (!ctx->switch_event) PARGO_HIP(ctx, hipEventCreateWithFlags(&ctx->switch_event, hipEventDisableTiming));
+    PARGO_HIP(ctx, hipEventRecord(ctx->switch_event, ctx->stream));
+    PARGO_HIP(ctx, hipStreamWaitEvent(ns, ctx->switch_event, 0));
+    ctx->stream = ns;
     return PARGO_OK;
 }
 

@@ -888,13 +888,17 @@ static void launch_fixup(const FocalParams &p, const FocalParams *tiles, int nti
     else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p, tiles, ntiles, width);
 }
 
-static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
-                         hipStream_t s)
+// The Horn fast path forms (nx^2 + ny^2) / (64 cs^2) in FP64 and takes the
+// root in FP32; with |z| < 2^40 (larger values go per cell) and
+// 2^-20 <= cs <= 2^40 that quotient stays below 2^127.
+static bool horn_ok(const FocalParams &p)
 {
-    // The Horn fast path forms (nx^2 + ny^2) / (64 cs^2) in FP64 and takes
-    // the root in FP32; with |z| < 2^40 (larger values go per cell) and
-    // 2^-20 <= cs <= 2^40 that quotient stays below 2^127.
-    const bool horn = p.method == PARGO_SLOPE_FD && p.cs >= 0x1p-20 && p.cs <= 0x1p40;
+    return p.method == PARGO_SLOPE_FD && p.cs >= 0x1p-20 && p.cs <= 0x1p40;
+}
+
+static void launch_focal_h(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
+                           bool horn, hipStream_t s)
+{
     if (do_slope && do_d8) {
         if (horn) launch_t<true, true, true>(p, cfg, vec, s);
         else launch_t<true, true, false>(p, cfg, vec, s);
@@ -904,6 +908,12 @@ static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, b
     } else {
         launch_t<false, true, true>(p, cfg, vec, s);
     }
+}
+
+static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
+                         hipStream_t s)
+{
+    launch_focal_h(p, cfg, vec, do_slope, do_d8, horn_ok(p), s);
 }
 
 static LaunchCfg default_cfg(int rows)
@@ -1157,9 +1167,12 @@ static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
     }
     b->do_slope = do_slope;
     b->do_d8 = do_d8;
-    b->horn = method == PARGO_SLOPE_FD;
+    // One slope path for the whole batch, in every mode: Horn's fast path
+    // only when every tile's cell size allows it (horn_ok), so that the
+    // per-tile, persistent and graph modes give the same bits.
+    b->horn = true;
     for (int t = 0; t < ntiles; t++)
-        if (!(b->h_tiles[t].cs >= 0x1p-20 && b->h_tiles[t].cs <= 0x1p40)) b->horn = false;
+        if (!horn_ok(b->h_tiles[t])) b->horn = false;
     // device order: largest tiles first
     std::vector<int> order(ntiles);
     for (int t = 0; t < ntiles; t++) order[t] = t;
@@ -1212,7 +1225,7 @@ static int run_batch(pargo_ctx *ctx, BatchPlan *b, int mode, int rows_per_strip)
         LaunchCfg cfg{b->cpt, rows_per_strip};
         for (int t = 0; t < b->ntiles; t++) {
             const FocalParams &p = b->h_tiles[t];
-            launch_focal(p, cfg, p.vec != 0, b->do_slope, b->do_d8, s);
+            launch_focal_h(p, cfg, p.vec != 0, b->do_slope, b->do_d8, b->horn, s);
             launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, b->cpt, s);
         }
         return check_launch(ctx, "pargo_batch_slope_d8 (per tile)");

@@ -429,6 +429,15 @@ __global__ void idw_unbucket_kernel(const double *__restrict__ acc, const int *_
     if (q < slots && perm[q] >= 0) out[perm[q]] = acc[q];
 }
 
+// v[slot] = vals[perm[slot]] (padding slots 0): new sample values in
+// bucket order, from a device array in file order
+__global__ void idw_bucket_values_kernel(const double *__restrict__ vals, const int *__restrict__ perm, int slots,
+                                         double *__restrict__ v)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < slots) v[q] = perm[q] >= 0 ? vals[perm[q]] : 0.0;
+}
+
 // Compute-load estimate of one coarse cell (IdwTransformation::Operator,
 // idwTransformation.cpp:3-62): the sample count of the cell's block, then
 // rings r = 0, 1, ... of which only the full top and bottom block rows are
@@ -682,6 +691,18 @@ int pargo_idw_set_values(pargo_ctx *ctx, pargo_idw_plan *plan, const double *val
     PARGO_HIP(ctx, hipMemcpyAsync(plan->d_v, v.data(), sizeof(double) * plan->slots, hipMemcpyHostToDevice, ctx->stream));
     PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PARGO_OK;
+}
+
+int pargo_idw_set_values_device(pargo_ctx *ctx, pargo_idw_plan *plan, const double *d_vals)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (!plan || !d_vals || plan->owner != ctx)
+        return fail(ctx, PARGO_E_INVALID, "pargo_idw_set_values_device: bad arguments");
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(pargo::idw_bucket_values_kernel, dim3((plan->slots + 255) / 256), dim3(256), 0, ctx->stream,
+                       d_vals, (const int *)plan->d_perm, plan->slots, plan->d_v);
+    return pargo::check_launch(ctx, "pargo_idw_set_values_device");
 }
 
 int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,

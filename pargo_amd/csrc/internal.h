@@ -18,6 +18,10 @@ struct pargo_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;  // where work goes (own or caller's)
+    // recorded on the old stream at a stream switch; the new stream waits
+    // for it, so the context's shared device state (fixup queue, scratch,
+    // batch queue) is never used by two streams at once
+    hipEvent_t switch_event = nullptr;
     std::string last_error;
     int num_cu = 0;
     // Small reusable device scratch owned by the context (grows on demand).

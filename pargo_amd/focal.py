@@ -25,18 +25,13 @@ def get_slope_algorithm(name):
 
 def _check_dem(dem):
     import torch
-    if not isinstance(dem, torch.Tensor) or dem.device.type != "cuda":
-        raise TypeError("dem must be a torch tensor on a HIP device")
-    if dem.dtype != torch.float32 or dem.dim() != 2:
-        raise TypeError("dem must be a 2-D float32 tensor")
-    if dem.stride(1) != 1:
-        raise ValueError("dem rows must be contiguous")
+    _lib.check_raster(dem, torch.float32, "dem")
 
 
 def grid_of(dem, cellsize, nodata):
     g = _lib.PargoGrid()
     g.rows, g.cols = int(dem.shape[0]), int(dem.shape[1])
-    g.pitch = int(dem.stride(0))
+    g.pitch = _lib.row_pitch(dem)
     g.cellsize = float(cellsize)
     g.nodata = float(nodata)
     return g
@@ -53,7 +48,7 @@ def _unit(unit):
 def _alloc_like(dem, dtype):
     import torch
     rows, cols = dem.shape
-    pitch = dem.stride(0)
+    pitch = _lib.row_pitch(dem)
     buf = torch.empty((rows, pitch), dtype=dtype, device=dem.device)
     return buf[:, :cols]
 

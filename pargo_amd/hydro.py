@@ -18,11 +18,7 @@ MODES = {"reference": _lib.SCA_REFERENCE, "converge": _lib.SCA_CONVERGE}
 
 
 def _check(t, dtype, name):
-    import torch
-    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
-        raise TypeError("%s must be a torch tensor on a HIP device" % name)
-    if t.dtype != dtype or t.dim() != 2 or t.stride(1) != 1:
-        raise TypeError("%s must be a 2-D %s tensor with contiguous rows" % (name, dtype))
+    _lib.check_raster(t, dtype, name)
 
 
 def d8_classify(d8, nodata, ctx=None):
@@ -32,7 +28,7 @@ def d8_classify(d8, nodata, ctx=None):
     ctx = ctx or _lib.default_context(d8.device.index or 0)
     ctx.bind_stream()
     rows, cols = d8.shape
-    buf = torch.empty((rows, d8.stride(0)), dtype=torch.uint8, device=d8.device)[:, :cols]
+    buf = torch.empty((rows, _lib.row_pitch(d8)), dtype=torch.uint8, device=d8.device)[:, :cols]
     g = grid_of(d8, 1.0, nodata)
     ctx.check(_lib.lib().pargo_d8_classify(ctx.h, ctypes.byref(g), _lib._ptr(d8), _lib._ptr(buf)))
     return buf
@@ -53,7 +49,7 @@ def sca_d8(flow, nodata, nodata_mask=None, mode="reference", ctx=None):
     ctx = ctx or _lib.default_context(flow.device.index or 0)
     ctx.bind_stream()
     rows, cols = flow.shape
-    out = torch.empty((rows, flow.stride(0)), dtype=torch.float64, device=flow.device)[:, :cols]
+    out = torch.empty((rows, _lib.row_pitch(flow)), dtype=torch.float64, device=flow.device)[:, :cols]
     g = grid_of(flow, 1.0, nodata)
     ctx.check(_lib.lib().pargo_sca_d8(ctx.h, ctypes.byref(g), _lib._ptr(flow), _lib._ptr(nodata_mask),
                                       MODES[mode], _lib._ptr(out)))
@@ -90,7 +86,7 @@ def pit_remove(dem, nodata, ctx=None):
     ctx = ctx or _lib.default_context(dem.device.index or 0)
     ctx.bind_stream()
     rows, cols = dem.shape
-    out = torch.empty((rows, dem.stride(0)), dtype=torch.float64, device=dem.device)[:, :cols]
+    out = torch.empty((rows, _lib.row_pitch(dem)), dtype=torch.float64, device=dem.device)[:, :cols]
     g = grid_of(dem, 1.0, nodata)
     passes = ctypes.c_int(0)
     ctx.check(_lib.lib().pargo_pit_remove(ctx.h, ctypes.byref(g), _lib._ptr(dem), _lib._ptr(out),

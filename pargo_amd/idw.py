@@ -66,19 +66,44 @@ class IDWOperator:
         return self
 
     def set_values(self, vals):
+        """New sample values in file order: a host array, or a float64
+        device tensor (then gathered on the device, stream-ordered, with no
+        host synchronisation)."""
+        import torch
+        if isinstance(vals, torch.Tensor) and vals.device.type == "cuda":
+            if vals.dtype != torch.float64 or vals.dim() != 1 or vals.numel() != self.n:
+                raise ValueError("device values must be a float64 vector of %d samples" % self.n)
+            vals = vals.contiguous()
+            self.ctx.bind_stream()
+            self.ctx.check(_lib.lib().pargo_idw_set_values_device(self.ctx.h, self.plan, _lib._ptr(vals)))
+            return
         vals = np.ascontiguousarray(vals, dtype=np.float64)
+        if vals.shape != (self.n,):
+            raise ValueError("values must hold %d samples" % self.n)
         self.ctx.check(_lib.lib().pargo_idw_set_values(self.ctx.h, self.plan, _lib._ptr(vals)))
 
     def mask_layer(self, mask, mask_nodata):
         """maskLayer: a device int32 grid; 0 and mask_nodata cells give -9999."""
-        self.mask = mask
+        import torch
+        _lib.check_raster(mask, torch.int32, "mask")
+        self.mask = mask.contiguous()
         self.mask_nodata = int(mask_nodata)
+
+    def _grid_tensor(self, t, dtype, name):
+        """A caller-supplied (rows, cols) device tensor of dtype, contiguous
+        (the kernels address it with pitch = cols)."""
+        import torch
+        if t is None:
+            return torch.empty((self.rows, self.cols), dtype=dtype, device=torch.device("cuda", self.ctx.device))
+        _lib.check_raster(t, dtype, name)
+        if tuple(t.shape) != (self.rows, self.cols) or not t.is_contiguous():
+            raise ValueError("%s must be a contiguous %dx%d tensor" % (name, self.rows, self.cols))
+        return t
 
     def run(self, out=None, count_candidates=False):
         import torch
         dev = torch.device("cuda", self.ctx.device)
-        if out is None:
-            out = torch.empty((self.rows, self.cols), dtype=torch.float32, device=dev)
+        out = self._grid_tensor(out, torch.float32, "out")
         cand = torch.zeros(1, dtype=torch.int64, device=dev) if count_candidates else None
         mr = mc = 0
         if self.mask is not None:
@@ -98,8 +123,9 @@ class IDWOperator:
         idwOperator.cpp:439-491).  out (float32, optional) gets the IDW
         values of the same run."""
         import torch
-        dev = torch.device("cuda", self.ctx.device)
-        work = torch.empty((self.rows, self.cols), dtype=torch.float64, device=dev)
+        if out is not None:
+            out = self._grid_tensor(out, torch.float32, "out")
+        work = self._grid_tensor(None, torch.float64, "work")
         mr = mc = 0
         if self.mask is not None:
             mr, mc = self.mask.shape
@@ -122,6 +148,8 @@ class IDWOperator:
         out = torch.empty((r.value, c.value), dtype=torch.float64, device=torch.device("cuda", self.ctx.device))
         mr = mc = 0
         if mask is not None:
+            _lib.check_raster(mask, torch.int32, "mask")
+            mask = mask.contiguous()
             mr, mc = mask.shape
         self.ctx.bind_stream()
         self.ctx.check(L.pargo_idw_load(self.ctx.h, self.plan, self.k, int(granularity), _lib._ptr(mask), mr, mc,
@@ -132,6 +160,7 @@ class IDWOperator:
     def adjoint(self, grad_out):
         """dL/d(sample values) in file order (float64 device tensor)."""
         import torch
+        grad_out = self._grid_tensor(grad_out.to(torch.float32).contiguous(), torch.float32, "grad_out")
         g = torch.empty(self.n, dtype=torch.float64, device=grad_out.device)
         mr = mc = 0
         if self.mask is not None:
@@ -139,7 +168,7 @@ class IDWOperator:
         self.ctx.bind_stream()
         self.ctx.check(_lib.lib().pargo_idw_adjoint(self.ctx.h, self.plan, self.k, self.power, self.buffer,
                                                     _lib._ptr(self.mask), mr, mc, self.mask_nodata,
-                                                    _lib._ptr(grad_out.contiguous()), _lib._ptr(g)))
+                                                    _lib._ptr(grad_out), _lib._ptr(g)))
         return g
 
     def free(self):

@@ -100,3 +100,33 @@ def test_autotune_keeps_results():
     for i in range(len(dems)):
         assert torch.equal(b.codes[i], ref[i][1])
         assert torch.equal(b.slopes[i], ref[i][0])
+
+
+def test_mixed_cell_sizes_all_modes_agree():
+    """A tile whose cell size is outside the Horn fast path's range sends the
+    whole batch down the generic FP64 path in every mode: the three modes
+    give the same bits, D8 equals single calls, slope stays within 1e-5."""
+    import torch
+    dems = make_tiles(12, 21, 8, 200)
+    sizes = [30.0] * len(dems)
+    sizes[4] = 1e-7  # below 2^-20
+    single = []
+    for d, cs in zip(dems, sizes):
+        s_, c_, _ = focal.slope_d8(d, cs, -9999.0)
+        single.append((s_.clone(), c_.clone()))
+    torch.cuda.synchronize()
+    outs = []
+    for mode in (batch.PER_TILE, batch.PERSISTENT, batch.GRAPH):
+        b = batch.Batch(dems, cellsize=sizes)
+        s, c = b.run(mode)
+        torch.cuda.synchronize()
+        outs.append(([x.clone() for x in s], [x.clone() for x in c]))
+    for m in (1, 2):
+        for i in range(len(dems)):
+            assert torch.equal(outs[m][0][i], outs[0][0][i]), "mode %d tile %d slope" % (m, i)
+            assert torch.equal(outs[m][1][i], outs[0][1][i]), "mode %d tile %d D8" % (m, i)
+    for i in range(len(dems)):
+        assert torch.equal(outs[0][1][i], single[i][1]), "tile %d D8 vs single call" % i
+        ref = single[i][0].double()
+        got = outs[0][0][i].double()
+        assert torch.all((got - ref).abs() <= 1e-5 * ref.abs() + 1e-12), "tile %d slope vs single call" % i
