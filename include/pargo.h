@@ -135,6 +135,21 @@ int pargo_slope_d8(pargo_ctx *ctx, const pargo_grid *grid, const float *dem,
                    int method, int unit, int dirtype, float *slope_out,
                    uint8_t *code_out, uint32_t *nodata_mask_out);
 
+/* pargo_slope_d8 without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, outputs and errors as pargo_slope_d8; the fixup queue it needs
+ * (two counters and 16384 strip entries, 262,400 bytes) lives in a static
+ * device array of the library, one per device, resolved by pargo_create.
+ * That fixed size covers every raster: strips beyond the 16384 entries are
+ * handled by the fixup pass's full scan, as in pargo_slope_d8.  Every call
+ * on one device shares the array, so calls that overlap on a device (two
+ * streams, two contexts, two threads) race; it is cleared by a kernel at the
+ * start of each call, never by a memset, so the call can be captured. */
+int pargo_slope_d8_noalloc(pargo_ctx *ctx, const pargo_grid *grid, const float *dem,
+                           int method, int unit, int dirtype, float *slope_out,
+                           uint8_t *code_out, uint32_t *nodata_mask_out);
+
 /* ---- many rasters ---------------------------------------------------- */
 
 /* One raster of a batch: its descriptor and device buffers.  All tiles of a

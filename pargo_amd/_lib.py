@@ -47,6 +47,7 @@ _SIGS = {
     "pargo_slope": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _P, _P]),
     "pargo_d8": (_c.c_int, [_P, _P, _P, _c.c_int, _P, _P]),
     "pargo_slope_d8": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _P, _P, _P]),
+    "pargo_slope_d8_noalloc": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _P, _P, _P]),
     "pargo_idw_grid": (_c.c_int, [_P, _P, _c.c_int, _c.c_float, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int),
                                   _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
     "pargo_idw_extent": (_c.c_int, [_P, _P, _c.c_int, _c.c_float, _P, _c.POINTER(_c.c_int),

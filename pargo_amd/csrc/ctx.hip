@@ -150,6 +150,13 @@ int pargo_create(int device, pargo_ctx **out)
         return fail(nullptr, PARGO_E_DEVICE, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
     }
     ctx->stream = ctx->own_stream;
+    // the static fixup queue of pargo_slope_d8_noalloc on this device
+    if (pargo::resolve_noalloc_scratch(ctx) != PARGO_OK) {
+        fail(nullptr, PARGO_E_DEVICE, "%s", ctx->last_error.c_str());
+        hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return PARGO_E_DEVICE;
+    }
     int cu = 0;
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess) ctx->num_cu = cu;
     *out = ctx;

@@ -949,11 +949,41 @@ static int ensure_fixup(pargo_ctx *ctx)
     return PARGO_OK;
 }
 
-static void attach_fixup(pargo_ctx *ctx, FocalParams &p)
+static void attach_fixup_at(void *mem, FocalParams &p)
 {
-    p.fix_count = (unsigned *)ctx->fix_mem;
-    p.fix_list = (uint4 *)((unsigned *)ctx->fix_mem + 64);
+    p.fix_count = (unsigned *)mem;
+    p.fix_list = (uint4 *)((unsigned *)mem + 64);
     p.fix_cap = kFixupCap;
+}
+
+static void attach_fixup(pargo_ctx *ctx, FocalParams &p) { attach_fixup_at(ctx->fix_mem, p); }
+
+// The fixup queue of pargo_slope_d8_noalloc, in a file-scope device array
+// instead of memory the context allocates: the same two counters (in a
+// 256-byte header) and kFixupCap strip entries, 262,400 bytes per device,
+// held until the process ends.  Every call on a device shares it.  Host code
+// never uses the array's name as an address: resolve_noalloc_scratch keeps
+// the device address hipGetSymbolAddress gives for the context's device.
+constexpr size_t kNoallocScratchBytes = 64 * sizeof(unsigned) + (size_t)kFixupCap * sizeof(uint4);
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+int resolve_noalloc_scratch(pargo_ctx *ctx)
+{
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(ctx, PARGO_E_DEVICE, "hipGetSymbolAddress of the static fixup queue on device %d: %s",
+                    ctx->device, hipGetErrorString(e));
+    ctx->noalloc_scratch = ptr;
+    return PARGO_OK;
+}
+
+// Zeroes the two queue counters (a kernel rather than a memset, so that a
+// caller may capture the call in a HIP graph).
+__global__ void fixup_counters_reset(unsigned *c)
+{
+    c[0] = 0;
+    c[1] = 0;
 }
 
 // Validates the request and fills the kernel parameters.
@@ -1012,7 +1042,7 @@ static int prepare(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int me
 }
 
 static int run_focal(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int method, int unit, int dirtype,
-                     float *slope_out, uint8_t *code_out, uint32_t *mask_out, const char *name)
+                     float *slope_out, uint8_t *code_out, uint32_t *mask_out, const char *name, bool noalloc = false)
 {
     FocalParams p;
     bool vec = false;
@@ -1026,9 +1056,17 @@ static int run_focal(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int 
     const bool do_slope = slope_out != nullptr;
     const bool do_d8 = code_out != nullptr;
     if (!do_slope && !do_d8) return fail(ctx, PARGO_E_INVALID, "%s: a NoData mask needs a slope or code output", name);
-    rc = ensure_fixup(ctx);
-    if (rc) return rc;
-    attach_fixup(ctx, p);
+    if (noalloc) {
+        // the static queue: no allocation; its counters may hold what an
+        // interrupted earlier call left, so they are cleared first
+        if (!ctx->noalloc_scratch) return fail(ctx, PARGO_E_DEVICE, "%s: the static fixup queue is unresolved", name);
+        attach_fixup_at(ctx->noalloc_scratch, p);
+        hipLaunchKernelGGL(fixup_counters_reset, dim3(1), dim3(1), 0, ctx->stream, p.fix_count);
+    } else {
+        rc = ensure_fixup(ctx);
+        if (rc) return rc;
+        attach_fixup(ctx, p);
+    }
     launch_focal(p, cfg, vec, do_slope, do_d8, ctx->stream);
     rc = check_launch(ctx, name);
     if (rc) return rc;
@@ -1276,6 +1314,13 @@ int pargo_slope_d8(pargo_ctx *ctx, const pargo_grid *grid, const float *dem, int
 {
     return pargo::run_focal(ctx, grid, dem, method, unit, dirtype, slope_out, code_out, nodata_mask_out,
                             "pargo_slope_d8");
+}
+
+int pargo_slope_d8_noalloc(pargo_ctx *ctx, const pargo_grid *grid, const float *dem, int method, int unit,
+                           int dirtype, float *slope_out, uint8_t *code_out, uint32_t *nodata_mask_out)
+{
+    return pargo::run_focal(ctx, grid, dem, method, unit, dirtype, slope_out, code_out, nodata_mask_out,
+                            "pargo_slope_d8_noalloc", true);
 }
 
 int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,

@@ -28,6 +28,9 @@ struct pargo_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     void *fix_mem = nullptr;  // focal fixup queue (counters + cell list)
+    // device address of the static fixup queue of pargo_slope_d8_noalloc on
+    // this context's device (resolved once by pargo_create)
+    void *noalloc_scratch = nullptr;
     // batched slope/D8: the last plan (with its captured graph) and the
     // launch shape chosen by pargo_batch_autotune
     pargo::BatchPlan *batch = nullptr;
@@ -38,6 +41,8 @@ struct pargo_ctx {
 
 namespace pargo {
 void free_batch_plan(pargo_ctx *ctx);
+// Resolves ctx->noalloc_scratch with ctx->device current.
+int resolve_noalloc_scratch(pargo_ctx *ctx);
 }
 
 namespace pargo {
