@@ -128,8 +128,8 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": 2 * 4 * n, "ms": ms, "GBs": 8 * n / ms / 1e6}), flush=True)
 
 
-CEIL_VARIANTS = {"strip": (0, 9), "strip_halo": (1, 9), "flat": (2, 9), "strip_nt": (3, 9), "read": (4, 4),
-                 "write": (5, 5)}
+CEIL_VARIANTS = {"strip": (0, 9), "strip_halo": (1, 9), "flat": (2, 9), "strip_nt": (3, 9), "flat_nt": (6, 9),
+                 "read": (4, 4), "write": (5, 5)}
 
 
 def secondary_ceiling(torch, size):
@@ -207,6 +207,18 @@ def secondary_idw(torch, synth, IDWOperator, cpu):
            "roofline": {"bound": "fp64 valu", "achieved": round(flop / ms / 1e9, 3), "peak": FP64_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(flop / ms / 1e9 / FP64_PEAK_TFLOPS, 4),
                         "flop_per_launch": flop}}
+    # tile launch order from the reference's load models (heaviest first):
+    # the values must not change, only the time
+    ref_out = out.clone()
+    orders = {}
+    for source in ("estimate", "measured"):
+        op.order_tiles(source)
+        o2 = op.run()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(o2, ref_out))
+        orders[source] = {"ms": round(event_time(torch, lambda: op.run(out=o2), 5, warm=1), 4), "values_identical": same}
+    op.order_tiles("rows")
+    res["tile_order"] = {"rows_ms": res["ms"], **orders}
     # the IDW adjoint (dL/d sample values) for an L2 loss against a seeded
     # target grid: dL/d(out) = 2 (out - target)
     gen = torch.Generator(device="cuda").manual_seed(synth.BASE_SEED + 3)
@@ -342,12 +354,21 @@ def secondary_batch(torch, batch, cpu):
     torch.cuda.synchronize()
     d8_same = all(torch.equal(b.codes[i], ref_c[i]) for i in range(len(dems)))
     slope_same = all(torch.equal(b.slopes[i], ref_s[i]) for i in range(len(dems)))
+    # queue order from the load models (the default is the estimate, cells)
+    orders = {"estimate_ms": round(gr, 3)}
+    for source in ("rows", "measured"):
+        b.order(source)
+        ms = event_time(torch, lambda: b.run(batch.GRAPH), 5, warm=1)
+        orders[source + "_ms"] = round(ms, 3)
+        orders[source + "_values_identical"] = all(torch.equal(b.codes[i], ref_c[i]) and
+                                                   torch.equal(b.slopes[i], ref_s[i]) for i in range(len(dems)))
+    b.order("estimate")
     res = {"config": "512 tiles, side exp(U[ln 256, ln 4096]), aspect U[0.5, 2], 1-5% NoData, slope+D8",
            "tiles": 512, "Gcells": round(cells / 1e9, 4), "total_bytes_in": 4 * cells,
            "total_bytes_algorithmic": 9 * cells, "per_tile_ms": round(per, 3), "graph_ms": round(gr, 3),
            "speedup": round(per / gr, 2), "graph_Gcells_s": round(cells / gr / 1e6, 1),
            "graph_GBs": round(9 * cells / gr / 1e6, 1), "autotuned": {"cols_per_lane": cpt, "rows_per_strip": rows},
-           "graph_equals_per_tile": {"d8": d8_same, "slope": slope_same}}
+           "graph_equals_per_tile": {"d8": d8_same, "slope": slope_same}, "queue_order": orders}
     if cpu is not None:
         from oracle import pyoracle
         # D8 and slope of every 32nd tile against the oracle

@@ -109,6 +109,11 @@ int pargo_sync(pargo_ctx *ctx);
 
 int pargo_alloc(pargo_ctx *ctx, size_t bytes, void **dptr);
 int pargo_free(pargo_ctx *ctx, void *dptr);
+/* Page-locked host memory (hipHostMalloc) for staging copies at full PCIe
+ * rate; the gpro layer stages rasters through it.  The reference moves
+ * rasters host-side only (GDAL RasterIO into CellSpace). */
+int pargo_alloc_host(pargo_ctx *ctx, size_t bytes, void **hptr);
+int pargo_free_host(pargo_ctx *ctx, void *hptr);
 int pargo_h2d(pargo_ctx *ctx, void *dst, const void *host_src, size_t bytes);
 int pargo_d2h(pargo_ctx *ctx, void *host_dst, const void *src, size_t bytes);
 
@@ -177,6 +182,23 @@ typedef struct {
  * (which takes the FP32-root path for them) and their D8 codes identical. */
 int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
                          int mode);
+
+/* Work orders from the reference's load models (batch queue, IDW tiles). */
+#define PARGO_ORDER_ROWS 0     /* as given / row-major */
+#define PARGO_ORDER_ESTIMATE 1 /* heaviest first by the estimated load */
+#define PARGO_ORDER_MEASURED 2 /* heaviest first by a measured pass */
+
+/* Queue order of later batches on ctx, heaviest tile first by a load
+ * model (the reference's ComputeLayer load modes, gpro/computeLayer.h:
+ * 378-443, which valRowDcmp turns into bands, gpro/deComposition.h:
+ * 305-393): PARGO_ORDER_ESTIMATE (the default) by cell count, the
+ * intensity-ratio load with equal NoData and valid weights, which is the
+ * fused kernel's per-cell cost; PARGO_ORDER_MEASURED by each tile's kernel
+ * time in one per-tile pass over `tiles` (the preliminary experiment);
+ * PARGO_ORDER_ROWS in the given order.  Outputs do not depend on the order.
+ * Synchronous. */
+int pargo_batch_order(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                      int source);
 
 /* Times the persistent launch for each candidate shape (columns per lane 2
  * or 4, rows per strip 16, 32, 64 or 128) on this batch, keeps the fastest for
@@ -283,6 +305,20 @@ int pargo_idw_set_values_device(pargo_ctx *ctx, pargo_idw_plan *plan, const doub
 int pargo_idw_run(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,
                   const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata,
                   float *out, unsigned long long *candidates_out);
+
+/* Launch order of IDW's 16 x 16 cell tiles for every later pargo_idw_run,
+ * pargo_idw_work and pargo_idw_adjoint on this plan, from the reference's
+ * load models (ComputeLayer::getCompuLoad, gpro/computeLayer.h:378-443):
+ * PARGO_ORDER_ROWS drops the order (row-major tiles); PARGO_ORDER_ESTIMATE
+ * launches the heaviest tiles first by the estimate of IdwTransformation
+ * (apps/spatial/idw/idwTransformation.cpp:3-62) over coarse cells of 16 x 16
+ * cells; PARGO_ORDER_MEASURED by the samples each tile's searches examine
+ * in one pass (the preliminary experiment, idwOperator.cpp:439-491).  On
+ * one GPU the load balance the reference gets from valRowDcmp's row bands
+ * (gpro/deComposition.h:305-393) becomes the order workgroups start in;
+ * values do not depend on it.  Synchronous. */
+int pargo_idw_order_tiles(pargo_ctx *ctx, pargo_idw_plan *plan, int k, int power, double buffer,
+                          const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, int source);
 
 /* Measured work of IDW per cell, the device form of the reference's
  * preliminary experiment: with -dcmp space -writeLoad the reference stores

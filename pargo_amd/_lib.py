@@ -42,6 +42,8 @@ _SIGS = {
     "pargo_sync": (_c.c_int, [_P]),
     "pargo_alloc": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_P)]),
     "pargo_free": (_c.c_int, [_P, _P]),
+    "pargo_alloc_host": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_P)]),
+    "pargo_free_host": (_c.c_int, [_P, _P]),
     "pargo_h2d": (_c.c_int, [_P, _P, _P, _c.c_size_t]),
     "pargo_d2h": (_c.c_int, [_P, _P, _P, _c.c_size_t]),
     "pargo_slope": (_c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _P, _P]),
@@ -58,6 +60,8 @@ _SIGS = {
     "pargo_idw_set_values_device": (_c.c_int, [_P, _P, _P]),
     "pargo_idw_run": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int, _c.c_int,
                                  _P, _P]),
+    "pargo_idw_order_tiles": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
+                                         _c.c_int, _c.c_int]),
     "pargo_idw_load": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
                                   _P, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
     "pargo_idw_work": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int, _c.c_int,
@@ -76,6 +80,7 @@ _SIGS = {
     "pargo_idw_adjoint": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _P, _c.c_int, _c.c_int,
                                      _c.c_int, _P, _P]),
     "pargo_batch_slope_d8": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "pargo_batch_order": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     "pargo_batch_autotune": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_int),
                                         _c.POINTER(_c.c_int)]),
     "pargo_synth_dem": (_c.c_int, [_c.c_uint64, _c.c_int, _c.c_int, _P, _c.c_int]),

@@ -7,6 +7,7 @@
 // written as a Float64 GeoTIFF; the reference reads the DEM without a
 // decomposition type, for which its writeFile returns without writing
 // (gpro/rasterLayer.h:1009-1012).
+#include <new>
 #include <cstdlib>
 #include <cstring>
 
@@ -31,7 +32,7 @@ static void Usage(const string &error_msg = "")
 }
 
 int main(int argc, char *argv[])
-{
+try {
     if (argc < 5) Usage("Too few arguments to run this program.");
     const char *inputfilename = nullptr, *neighborfile = nullptr, *outputfilename = nullptr;
     int dirType = 0;
@@ -82,4 +83,8 @@ int main(int argc, char *argv[])
     cout << "run time is " << endtime - starttime << endl;
     if (!sflowLayer.writeFile(outputfilename)) return 1;
     return Application::END() ? 0 : 1;
+} catch (const std::bad_alloc &) {
+    // the reference stops with a message where an allocation fails
+    cerr << "[ERROR] out of host memory" << endl;
+    return 1;
 }

@@ -11,6 +11,7 @@
 // reference records CPU milliseconds; the device records the number of
 // samples each cell's search examined (its work, deterministic), which
 // -dcmp compute -readLoad balances the same way.
+#include <new>
 #include <cstdlib>
 #include <cstring>
 
@@ -38,7 +39,7 @@ static void Usage(const string &error_msg = "")
 }
 
 int main(int argc, char *argv[])
-{
+try {
     const char *inputFileName = nullptr, *maskFileName = nullptr, *outputFileName = nullptr;
     const char *writeLoadPath = nullptr, *readLoadPath = nullptr;
     float cellSize = 0;
@@ -132,4 +133,8 @@ int main(int argc, char *argv[])
     if (!idwLayer.writeFile(outputFileName)) return 1;
     cout << "write done." << endl;
     return Application::END() ? 0 : 1;
+} catch (const std::bad_alloc &) {
+    // the reference stops with a message where an allocation fails
+    cerr << "[ERROR] out of host memory" << endl;
+    return 1;
 }

@@ -5,6 +5,7 @@
 //   pitremove <dem> <nbr> <filled>
 // The filled DEM is written as a Float64 GeoTIFF with the DEM's
 // georeferencing and NoData.
+#include <new>
 #include <cstdlib>
 #include <cstring>
 
@@ -27,7 +28,7 @@ static void Usage(const string &error_msg = "")
 }
 
 int main(int argc, char *argv[])
-{
+try {
     if (argc < 4) Usage("Too few arguments to run this program.");
     const char *inputfilename = nullptr, *neighborfile = nullptr, *outputfilename = nullptr;
     bool simpleusage = true;
@@ -72,4 +73,8 @@ int main(int argc, char *argv[])
 
     if (!wdemLayer.writeFile(outputfilename)) return 1;
     return Application::END() ? 0 : 1;
+} catch (const std::bad_alloc &) {
+    // the reference stops with a message where an allocation fails
+    cerr << "[ERROR] out of host memory" << endl;
+    return 1;
 }

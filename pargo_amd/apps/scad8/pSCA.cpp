@@ -9,12 +9,13 @@
 // written as a Float64 GeoTIFF with the D8 layer's georeferencing and
 // NoData; the reference reads the D8 file without a decomposition type, for
 // which its writeFile returns without writing (gpro/rasterLayer.h:1009-1012).
+#include <new>
 #include <cstring>
 
 #include "scaOperator.h"
 
 int main(int argc, char *argv[])
-{
+try {
     bool converge = false;
     if (argc == 5 && !strcmp(argv[4], "-converge")) {
         converge = true;
@@ -44,4 +45,8 @@ int main(int argc, char *argv[])
 
     if (!scaLayer.writeFile(outputfilename)) return 1;
     return Application::END() ? 0 : 1;
+} catch (const std::bad_alloc &) {
+    // the reference stops with a message where an allocation fails
+    cerr << "[ERROR] out of host memory" << endl;
+    return 1;
 }

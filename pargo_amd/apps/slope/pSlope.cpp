@@ -1,6 +1,7 @@
 // pSlope.cpp -- the slope command line (apps/morphology/slope/pSlope.cpp of
 // the reference, main :52-154): same flags, same simple usage, same output
 // (tan of the slope angle as a Float64 GeoTIFF with the DEM's NoData).
+#include <new>
 #include <cstring>
 
 #include "slopeOperator.h"
@@ -26,7 +27,7 @@ static void Usage(const string &error_msg = "")
 }
 
 int main(int argc, char *argv[])
-{
+try {
     if (argc < 4) Usage("Too few arguments to run this program.");
     const char *inputfilename = nullptr, *neighborfile = nullptr, *outputfilename = nullptr;
     SlopeAlgor calcalgor = FD;
@@ -73,4 +74,8 @@ int main(int argc, char *argv[])
     cout << "run time is " << endtime - starttime << endl;
     if (!slopeLayer.writeFile(outputfilename)) return 1;
     return Application::END() ? 0 : 1;
+} catch (const std::bad_alloc &) {
+    // the reference stops with a message where an allocation fails
+    cerr << "[ERROR] out of host memory" << endl;
+    return 1;
 }

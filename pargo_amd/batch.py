@@ -66,6 +66,16 @@ class Batch:
                                                    len(self.dems), self.method, self.unit, self.dirtype, int(mode)))
         return self.slopes, self.codes
 
+    ORDER = {"rows": 0, "estimate": 1, "measured": 2}
+
+    def order(self, source="estimate"):
+        """Queue order of later runs: heaviest tile first by cell count
+        ("estimate"), by per-tile kernel time measured in one pass
+        ("measured"), or as given ("rows")."""
+        self.ctx.bind_stream()
+        self.ctx.check(self.L.pargo_batch_order(self.ctx.h, ctypes.cast(self.tiles, ctypes.c_void_p), len(self.dems),
+                                                self.method, self.unit, self.dirtype, self.ORDER[source]))
+
     def autotune(self):
         self.ctx.bind_stream()
         c, r = ctypes.c_int(), ctypes.c_int()

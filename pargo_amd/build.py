@@ -36,7 +36,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-ffp-contract=off"]
 
 HIP_SOURCES = ["ctx.hip", "focal.hip", "idw.hip", "adjoint.hip", "sca.hip", "fill.hip", "compute.hip"]
-CXX_SOURCES = ["synth.cpp", "dcmp.cpp", "io.cpp", "../gpro/geotiff.cpp", "../gpro/shapefile.cpp"]
+CXX_SOURCES = ["synth.cpp", "dcmp.cpp", "io.cpp", "faults.cpp", "../gpro/geotiff.cpp", "../gpro/shapefile.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -82,7 +82,7 @@ def build_ceiling():
 
 GPRO = os.path.join(PKG, "gpro")
 # the GeoTIFF and shapefile codecs are part of libpargo.so (pargo_geotiff_*)
-GPRO_SOURCES = ["application.cpp", "focalDevice.cpp", "hydroDevice.cpp"]
+GPRO_SOURCES = ["application.cpp", "focalDevice.cpp", "hydroDevice.cpp", "deviceRaster.cpp"]
 # command line -> (app directory, sources)
 APP_SOURCES = {
     "slope": ("slope", ["slopeOperator.cpp", "pSlope.cpp"]),
@@ -92,6 +92,10 @@ APP_SOURCES = {
     "pitremove": ("pitremove", ["pitRemoveOperator.cpp", "pPitRemove.cpp"]),
     # host-only checks of the gpro layer, run by tests/test_gpro_host.py
     "gpro_host_test": ("../../tests/cpp", ["gpro_host_test.cpp"]),
+    # GPU checks of the device-resident layers, run by tests/test_gpu_gpro_chain.py
+    "gpro_device_chain": ("../../tests/cpp", ["gpro_device_chain.cpp", "../../pargo_amd/apps/flowdird8/d8Operator.cpp",
+                                              "../../pargo_amd/apps/scad8/scaOperator.cpp",
+                                              "../../pargo_amd/apps/slope/slopeOperator.cpp"]),
 }
 APP_FLAGS = ["-O2", "-std=c++17", "-fopenmp", "-Wall", "-ffp-contract=off"]
 

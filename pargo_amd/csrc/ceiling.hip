@@ -13,6 +13,9 @@
 //   STRIP_HALO  the same, also reading the row above and below each strip
 //               (the focal kernel's halo rows)
 //   FLAT        one grid-stride loop over the cells, 8 workgroups per CU
+//   FLAT_NT     the same with non-temporal stores and 4 workgroups per CU:
+//               the whole GPU sweeps one narrow window of the raster (the
+//               fastest form measured for this traffic mix)
 //   STRIP_NT    STRIP with non-temporal stores
 //   READ        STRIP reading only (4 B/cell)
 //   WRITE       STRIP writing only (5 B/cell)
@@ -30,6 +33,7 @@
 #define PARGO_CEIL_STRIP_NT 3
 #define PARGO_CEIL_READ 4
 #define PARGO_CEIL_WRITE 5
+#define PARGO_CEIL_FLAT_NT 6
 
 namespace {
 
@@ -88,6 +92,7 @@ __global__ __launch_bounds__(256) void strip_kernel(const float *__restrict__ sr
     if (!WRITE && acc == -1.0f) sink[0] = 1u;
 }
 
+template <bool NT>
 __global__ __launch_bounds__(256) void flat_kernel(const float *__restrict__ src, float *__restrict__ dst,
                                                    uint8_t *__restrict__ dst8, int rows, int cols, long long pitch)
 {
@@ -98,8 +103,14 @@ __global__ __launch_bounds__(256) void flat_kernel(const float *__restrict__ src
         const long long r = (long long)(g / per_row);
         const unsigned c = (unsigned)(g % per_row) * 4u;
         const float4 v = *reinterpret_cast<const float4 *>(src + r * pitch + c);
-        *reinterpret_cast<float4 *>(dst + r * pitch + c) = v;
-        *reinterpret_cast<unsigned *>(dst8 + r * pitch + c) = pack_codes(v);
+        if (NT) {
+            const f4v t = {v.x, v.y, v.z, v.w};
+            __builtin_nontemporal_store(t, reinterpret_cast<f4v *>(dst + r * pitch + c));
+            __builtin_nontemporal_store(pack_codes(v), reinterpret_cast<unsigned *>(dst8 + r * pitch + c));
+        } else {
+            *reinterpret_cast<float4 *>(dst + r * pitch + c) = v;
+            *reinterpret_cast<unsigned *>(dst8 + r * pitch + c) = pack_codes(v);
+        }
     }
 }
 
@@ -139,12 +150,18 @@ int pargo_ceiling_launch(int variant, const float *src, float *dst, uint8_t *dst
         hipLaunchKernelGGL((strip_kernel<false, true, false, false>), grid, block, 0, s, src, dst, dst8, rows, pitch,
                            rows_per_strip, sink);
         break;
-    case PARGO_CEIL_FLAT: {
+    case PARGO_CEIL_FLAT:
+    case PARGO_CEIL_FLAT_NT: {
         int dev = 0, cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu < 1)
             cu = 256;
-        hipLaunchKernelGGL(flat_kernel, dim3((unsigned)(cu * 8)), block, 0, s, src, dst, dst8, rows, cols, pitch);
+        if (variant == PARGO_CEIL_FLAT)
+            hipLaunchKernelGGL(flat_kernel<false>, dim3((unsigned)(cu * 8)), block, 0, s, src, dst, dst8, rows, cols,
+                               pitch);
+        else
+            hipLaunchKernelGGL(flat_kernel<true>, dim3((unsigned)(cu * 4)), block, 0, s, src, dst, dst8, rows, cols,
+                               pitch);
         break;
     }
     default:

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 
+#include "faults.h"
 #include "internal.h"
 
 namespace pargo {
@@ -46,6 +47,24 @@ int check_launch(pargo_ctx *ctx, const char *what)
     return PARGO_OK;
 }
 
+hipError_t dev_malloc(void **ptr, size_t bytes, const char *site)
+{
+    if (alloc_should_fail(site)) {
+        *ptr = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(ptr, bytes);
+}
+
+const char *injected_note()
+{
+    const char *s = take_injected_site();
+    static thread_local char buf[160];
+    if (!*s) return "";
+    snprintf(buf, sizeof(buf), " (injected at %s)", s);
+    return buf;
+}
+
 int ensure_scratch(pargo_ctx *ctx, size_t bytes)
 {
     if (ctx->scratch_bytes >= bytes) return PARGO_OK;
@@ -58,8 +77,9 @@ int ensure_scratch(pargo_ctx *ctx, size_t bytes)
         ctx->scratch_bytes = 0;
     }
     size_t want = bytes < 4096 ? 4096 : bytes;
-    hipError_t e = hipMalloc(&ctx->scratch, want);
-    if (e != hipSuccess) return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for scratch: %s", want, hipGetErrorString(e));
+    hipError_t e = dev_malloc(&ctx->scratch, want, "context scratch");
+    if (e != hipSuccess)
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for scratch: %s%s", want, hipGetErrorString(e), injected_note());
     ctx->scratch_bytes = want;
     return PARGO_OK;
 }
@@ -135,8 +155,8 @@ int pargo_create(int device, pargo_ctx **out)
     if (e != hipSuccess) return fail(nullptr, PARGO_E_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
     if (n <= 0) return fail(nullptr, PARGO_E_DEVICE, "no HIP device");
     if (device < 0 || device >= n) return fail(nullptr, PARGO_E_INVALID, "device %d not in [0, %d)", device, n);
-    pargo_ctx *ctx = new (std::nothrow) pargo_ctx();
-    if (!ctx) return fail(nullptr, PARGO_E_NOMEM, "out of host memory");
+    pargo_ctx *ctx = pargo::alloc_should_fail("context") ? nullptr : new (std::nothrow) pargo_ctx();
+    if (!ctx) return fail(nullptr, PARGO_E_NOMEM, "out of host memory%s", pargo::injected_note());
     ctx->device = device;
     e = hipSetDevice(device);
     if (e != hipSuccess) {
@@ -216,8 +236,9 @@ int pargo_alloc(pargo_ctx *ctx, size_t bytes, void **dptr)
     if (!ctx || !dptr) return fail(ctx, PARGO_E_INVALID, "pargo_alloc: null argument");
     int rc = pargo::bind(ctx);
     if (rc) return rc;
-    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
-    if (e != hipSuccess) return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    hipError_t e = pargo::dev_malloc(dptr, bytes ? bytes : 1, "pargo_alloc");
+    if (e != hipSuccess)
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu): %s%s", bytes, hipGetErrorString(e), pargo::injected_note());
     return PARGO_OK;
 }
 
@@ -227,6 +248,30 @@ int pargo_free(pargo_ctx *ctx, void *dptr)
     int rc = pargo::bind(ctx);
     if (rc) return rc;
     if (dptr) PARGO_HIP(ctx, hipFree(dptr));
+    return PARGO_OK;
+}
+
+int pargo_alloc_host(pargo_ctx *ctx, size_t bytes, void **hptr)
+{
+    if (!ctx || !hptr) return fail(ctx, PARGO_E_INVALID, "pargo_alloc_host: null argument");
+    *hptr = nullptr;
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    hipError_t e = pargo::alloc_should_fail("pargo_alloc_host") ? hipErrorOutOfMemory
+                                                                : hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *hptr = nullptr;
+        return fail(ctx, PARGO_E_NOMEM, "hipHostMalloc(%zu): %s%s", bytes, hipGetErrorString(e), pargo::injected_note());
+    }
+    return PARGO_OK;
+}
+
+int pargo_free_host(pargo_ctx *ctx, void *hptr)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    if (hptr) PARGO_HIP(ctx, hipHostFree(hptr));
     return PARGO_OK;
 }
 

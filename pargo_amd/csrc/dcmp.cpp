@@ -3,23 +3,20 @@
 #include <vector>
 
 #include "../../include/pargo.h"
-
-namespace pargo {
-int fail(pargo_ctx *ctx, int code, const char *fmt, ...);
-}
+#include "faults.h"
 
 namespace {
 constexpr double kEps = 0.0000001;  // gpro/deComposition.h:24
 }
 
-extern "C" int pargo_load_row_bands(const double *load, int rows, int cols, double nodata, int nbands,
-                                    int *band_first)
+static int load_row_bands(const double *load, int rows, int cols, double nodata, int nbands, int *band_first)
 {
     if (!load || !band_first || rows < 1 || cols < 1)
         return pargo::fail(nullptr, PARGO_E_INVALID, "pargo_load_row_bands: bad arguments");
     if (nbands < 1 || nbands > rows)
         return pargo::fail(nullptr, PARGO_E_INVALID, "invalid number of SubSpaces (%d) for %d rows", nbands, rows);
     // per-row load over the cells that are neither -9999 nor NoData
+    pargo::alloc_point("bands.row_loads");
     std::vector<double> rowLoad(rows, 0.0);
     double total = 0.0;
     for (int r = 0; r < rows; r++) {
@@ -57,4 +54,11 @@ extern "C" int pargo_load_row_bands(const double *load, int rows, int cols, doub
                                          nbands - 1);  // unreachable: every pass places one cut
     band_first[nbands] = rows;
     return PARGO_OK;
+}
+
+extern "C" int pargo_load_row_bands(const double *load, int rows, int cols, double nodata, int nbands,
+                                    int *band_first)
+{
+    return pargo::guard(nullptr, "pargo_load_row_bands",
+                        [&]() -> int { return load_row_bands(load, rows, cols, nodata, nbands, band_first); });
 }

@@ -940,9 +940,10 @@ static int ensure_fixup(pargo_ctx *ctx)
 {
     if (ctx->fix_mem) return PARGO_OK;
     const size_t bytes = 64 * sizeof(unsigned) + (size_t)kFixupCap * sizeof(uint4);
-    hipError_t e = hipMalloc(&ctx->fix_mem, bytes);
-    if (e != hipSuccess) return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the fixup queue: %s", bytes,
-                                     hipGetErrorString(e));
+    hipError_t e = dev_malloc(&ctx->fix_mem, bytes, "fixup queue");
+    if (e != hipSuccess)
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the fixup queue: %s%s", bytes, hipGetErrorString(e),
+                    injected_note());
     e = hipMemsetAsync(ctx->fix_mem, 0, 64 * sizeof(unsigned), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(ctx, PARGO_E_DEVICE, "clearing the fixup queue: %s", hipGetErrorString(e));
@@ -1165,9 +1166,9 @@ static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
 {
     if (!tiles || ntiles < 1) return fail(ctx, PARGO_E_INVALID, "empty batch");
     if (ntiles > (1 << 24)) return fail(ctx, PARGO_E_INVALID, "too many tiles");
-    std::vector<unsigned char> key(sizeof(pargo_tile) * (size_t)ntiles + 6 * sizeof(int));
+    std::vector<unsigned char> key(sizeof(pargo_tile) * (size_t)ntiles + 7 * sizeof(int));
     std::memcpy(key.data(), tiles, sizeof(pargo_tile) * (size_t)ntiles);
-    const int extra[6] = {ntiles, method, unit, dirtype, cpt, rows_per_strip};
+    const int extra[7] = {ntiles, method, unit, dirtype, cpt, rows_per_strip, ctx->batch_order};
     std::memcpy(key.data() + sizeof(pargo_tile) * (size_t)ntiles, extra, sizeof(extra));
     if (ctx->batch && ctx->batch->key == key) {
         *out = ctx->batch;
@@ -1176,6 +1177,7 @@ static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
     free_batch_plan(ctx);
     int rc = ensure_fixup(ctx);
     if (rc) return rc;
+    alloc_point("batch plan (host)");
     BatchPlan *b = new (std::nothrow) BatchPlan();
     if (!b) return fail(ctx, PARGO_E_NOMEM, "out of host memory");
     b->key = std::move(key);
@@ -1211,12 +1213,21 @@ static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
     b->horn = true;
     for (int t = 0; t < ntiles; t++)
         if (!horn_ok(b->h_tiles[t])) b->horn = false;
-    // device order: largest tiles first
+    // device order, heaviest first by the load model of ctx->batch_order:
+    // cells (the intensity-ratio load with equal NoData and valid weights,
+    // which is this kernel's cost: a NoData cell costs what a valid one
+    // does), per-tile kernel times measured in one pass, or the given order
     std::vector<int> order(ntiles);
     for (int t = 0; t < ntiles; t++) order[t] = t;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-        return (long long)tiles[a].grid.rows * tiles[a].grid.cols > (long long)tiles[c].grid.rows * tiles[c].grid.cols;
-    });
+    if (ctx->batch_order == PARGO_ORDER_MEASURED && (int)ctx->batch_load.size() == ntiles) {
+        const std::vector<double> &ld = ctx->batch_load;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return ld[a] > ld[c]; });
+    } else if (ctx->batch_order != PARGO_ORDER_ROWS) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            return (long long)tiles[a].grid.rows * tiles[a].grid.cols >
+                   (long long)tiles[c].grid.rows * tiles[c].grid.cols;
+        });
+    }
     std::vector<FocalParams> dev(ntiles);
     std::vector<unsigned> first(ntiles);
     unsigned long long total = 0;
@@ -1234,10 +1245,10 @@ static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
     b->total = (unsigned)total;
     const size_t bt = sizeof(FocalParams) * ntiles, bf = sizeof(unsigned) * ntiles;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    hipError_t e = hipMalloc(&b->d_mem, al(bt) + al(bf) + 256);
+    hipError_t e = dev_malloc(&b->d_mem, al(bt) + al(bf) + 256, "batch plan");
     if (e != hipSuccess) {
         delete b;
-        return fail(ctx, PARGO_E_NOMEM, "hipMalloc for the batch plan: %s", hipGetErrorString(e));
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc for the batch plan: %s%s", hipGetErrorString(e), injected_note());
     }
     b->d_tiles = (FocalParams *)b->d_mem;
     b->d_first = (unsigned *)((char *)b->d_mem + al(bt));
@@ -1329,16 +1340,18 @@ int pargo_batch_slope_d8(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, in
     if (!ctx) return PARGO_E_INVALID;
     if (mode < PARGO_BATCH_PER_TILE || mode > PARGO_BATCH_GRAPH)
         return pargo::fail(ctx, PARGO_E_INVALID, "unknown batch mode %d", mode);
-    int rc = pargo::bind(ctx);
-    if (rc) return rc;
-    pargo::BatchPlan *b = nullptr;
-    rc = pargo::build_batch(ctx, tiles, ntiles, method, unit, dirtype, ctx->batch_cpt, ctx->batch_rows, &b);
-    if (rc) return rc;
-    return pargo::run_batch(ctx, b, mode, ctx->batch_rows);
+    return pargo::guard(ctx, "pargo_batch_slope_d8", [&]() -> int {
+        int rc = pargo::bind(ctx);
+        if (rc) return rc;
+        pargo::BatchPlan *b = nullptr;
+        rc = pargo::build_batch(ctx, tiles, ntiles, method, unit, dirtype, ctx->batch_cpt, ctx->batch_rows, &b);
+        if (rc) return rc;
+        return pargo::run_batch(ctx, b, mode, ctx->batch_rows);
+    });
 }
 
-int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
-                         int *cols_per_lane, int *rows_per_strip)
+static int batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                          int *cols_per_lane, int *rows_per_strip)
 {
     if (!ctx) return PARGO_E_INVALID;
     int rc = pargo::bind(ctx);
@@ -1377,6 +1390,66 @@ int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, in
     if (cols_per_lane) *cols_per_lane = bc;
     if (rows_per_strip) *rows_per_strip = br;
     return PARGO_OK;
+}
+
+int pargo_batch_autotune(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                         int *cols_per_lane, int *rows_per_strip)
+{
+    return pargo::guard(ctx, "pargo_batch_autotune", [&]() -> int {
+        return batch_autotune(ctx, tiles, ntiles, method, unit, dirtype, cols_per_lane, rows_per_strip);
+    });
+}
+
+static int batch_order(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                       int source)
+{
+    if (!ctx) return PARGO_E_INVALID;
+    if (source != PARGO_ORDER_ROWS && source != PARGO_ORDER_ESTIMATE && source != PARGO_ORDER_MEASURED)
+        return pargo::fail(ctx, PARGO_E_INVALID, "unknown batch order source %d", source);
+    int rc = pargo::bind(ctx);
+    if (rc) return rc;
+    ctx->batch_load.clear();
+    if (source == PARGO_ORDER_MEASURED) {
+        // the preliminary experiment: one per-tile pass with an event pair
+        // around every tile's launches
+        pargo::BatchPlan *b = nullptr;
+        rc = pargo::build_batch(ctx, tiles, ntiles, method, unit, dirtype, ctx->batch_cpt, ctx->batch_rows, &b);
+        if (rc) return rc;
+        pargo::alloc_point("batch tile loads (host)");
+        std::vector<hipEvent_t> ev(2 * (size_t)ntiles, nullptr);
+        hipError_t e = hipSuccess;
+        for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+        const pargo::LaunchCfg cfg{b->cpt, ctx->batch_rows};
+        for (int t = 0; t < ntiles && e == hipSuccess; t++) {
+            const pargo::FocalParams &p = b->h_tiles[t];
+            e = hipEventRecord(ev[2 * t], ctx->stream);
+            pargo::launch_focal_h(p, cfg, p.vec != 0, b->do_slope, b->do_d8, b->horn, ctx->stream);
+            pargo::launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, b->cpt, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[2 * t + 1], ctx->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        std::vector<double> load((size_t)ntiles, 0.0);
+        for (int t = 0; t < ntiles && e == hipSuccess; t++) {
+            float ms = 0;
+            e = hipEventElapsedTime(&ms, ev[2 * t], ev[2 * t + 1]);
+            load[t] = ms;
+        }
+        for (hipEvent_t x : ev)
+            if (x) hipEventDestroy(x);
+        if (e != hipSuccess) return pargo::fail(ctx, PARGO_E_DEVICE, "timing the batch tiles: %s", hipGetErrorString(e));
+        ctx->batch_load = std::move(load);
+    }
+    ctx->batch_order = source;
+    pargo::free_batch_plan(ctx);  // the next batch call builds the queue in the new order
+    return PARGO_OK;
+}
+
+int pargo_batch_order(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int method, int unit, int dirtype,
+                      int source)
+{
+    return pargo::guard(ctx, "pargo_batch_order", [&]() -> int {
+        return batch_order(ctx, tiles, ntiles, method, unit, dirtype, source);
+    });
 }
 
 }  // extern "C"

@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <memory>
 #include <vector>
 
 #include "internal.h"
@@ -61,6 +62,22 @@ struct pargo_idw_plan {
     int *d_perm = nullptr;     // bucket slot -> file index (-1: padding)
     int slots = 0;             // padded slot count
     std::vector<int> h_perm;
+    // launch order of the 16 x 16 cell tiles, heaviest estimated or
+    // measured load first (pargo_idw_order_tiles); null: row-major order
+    int *d_order = nullptr;
+    int order_source = 0;
+};
+
+// Frees a plan with its device memory (after the work queued on it).
+struct PlanDeleter {
+    void operator()(pargo_idw_plan *p) const
+    {
+        if (!p) return;
+        if (p->owner) hipStreamSynchronize(p->owner->stream);
+        if (p->d_mem) hipFree(p->d_mem);
+        if (p->d_order) hipFree(p->d_order);
+        delete p;
+    }
 };
 
 namespace pargo {
@@ -101,6 +118,8 @@ struct IdwParams {
     unsigned long long *__restrict__ cand_out;
     const float *__restrict__ grad_out;  // adjoint: dL/d(out), rows x cols
     double *__restrict__ grad_acc;       // adjoint: per bucket slot
+    const int *__restrict__ order;       // tile launch order (null: row-major)
+    double *__restrict__ tile_work;      // per tile: samples examined (may be null)
     double minX, maxY, block, buffer, cs;
     int rows, cols, blockRows, blockCols, maxRing;
     int k, power;
@@ -303,6 +322,178 @@ __device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow
     return (tail + 1 < K) ? tail + 1 : K;
 }
 
+// The same search with each ring's candidates visited nearest-first, for
+// the lanes with `mine` set.  The reference's result is the K smallest
+// distances among the samples of the visited rings, whatever order they are
+// visited in, except when a sample outside the list ties the list's largest
+// distance exactly: then its visiting order picks the member.  So the
+// visited rings are the reference's (same stop tests, which depend only on
+// the K-th distance and the sample count), each ring's candidates are
+// staged across the wave (candidate i of the ring, blocks in ring order and
+// slots in file order, in lane i % 64, register i / 64) and visited in bins
+// of increasing distance from the group's cells, and the smallest distance
+// seen outside the list is kept: `tie` reports a lane whose list depends on
+// the order, which the caller then searches again in the reference's order.
+// A bin's candidates are visited only while some lane's threshold reaches
+// the bin's inner edge, so far candidates are never loaded into a test.
+constexpr int kStage = 4;     // staged candidates per lane: 256 per wave
+constexpr int kBins = 48;     // bins of block / 16 each, the last one open
+constexpr int kBinBits = 6;   // a staged entry packs (slot << 6) | bin
+
+__device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRow, int blockCol, double cellX,
+                                                 double cellY, bool mine, double *kd, int *ks, unsigned &ncand,
+                                                 bool &tie)
+{
+    const int K = p.k;
+    const int lane = threadIdx.x & 63;
+    const double bx0 = blockCol * p.block + p.minX, bx1 = (blockCol + 1) * p.block + p.minX;
+    const double by0 = p.maxY - blockRow * p.block, by1 = p.maxY - (blockRow + 1) * p.block;
+    const double d0 = fabs(cellX - bx0), d1 = fabs(cellX - bx1), d2 = fabs(cellY - by0), d3 = fabs(cellY - by1);
+    const double a = (d1 < d0) ? d1 : d0, b = (d3 < d2) ? d3 : d2;
+    const double minDistToBound = (b < a) ? b : a;
+
+    double maxDist = 0.0;
+    int maxIdx = -1;
+    int tail = -1;
+    bool act = mine;
+    double kthr = act ? INFINITY : -INFINITY;
+    double runner = INFINITY;  // smallest distance visited outside the list
+    // bounding box of the group's cells (every lane's cell lies inside it)
+    double gx0 = mine ? cellX : INFINITY, gx1 = mine ? cellX : -INFINITY;
+    double gy0 = mine ? cellY : INFINITY, gy1 = mine ? cellY : -INFINITY;
+    for (int o = 32; o >= 1; o >>= 1) {
+        gx0 = fmin(gx0, __shfl_xor(gx0, o, 64));
+        gx1 = fmax(gx1, __shfl_xor(gx1, o, 64));
+        gy0 = fmin(gy0, __shfl_xor(gy0, o, 64));
+        gy1 = fmax(gy1, __shfl_xor(gy1, o, 64));
+    }
+    const float inv_w = fminf((float)(16.0 / p.block), 3.4e38f);  // finite: no 0 * inf in the bins
+    const double w = p.block / 16.0;
+
+    auto visit = [&](double2 xy, double pre, int q) {
+        if (pre <= kthr) {
+            const double dx = xy.x - cellX, dy = xy.y - cellY;
+            const double s = dx * dx + dy * dy;
+            const double tmpDist = (s == 0.0) ? kIdwEps : sqrt(s);
+            if (tail < K - 1) {
+                tail++;
+                kd[tail * 256] = tmpDist;
+                ks[tail * 256] = q;
+                if (tmpDist > maxDist) {
+                    maxDist = tmpDist;
+                    maxIdx = tail;
+                }
+                if (tail == K - 1) kthr = maxDist * maxDist * kPrefilterSlack;
+            } else if (tmpDist < maxDist) {
+                runner = fmin(runner, maxDist);  // the evicted distance
+                kd[maxIdx * 256] = tmpDist;
+                ks[maxIdx * 256] = q;
+                maxDist = kd[0];
+                maxIdx = 0;
+                for (int i = 1; i < K; i++) {
+                    const double di = kd[i * 256];
+                    if (di > maxDist) {
+                        maxDist = di;
+                        maxIdx = i;
+                    }
+                }
+                kthr = maxDist * maxDist * kPrefilterSlack;
+            } else {
+                runner = fmin(runner, tmpDist);
+            }
+        }
+    };
+
+    // the staging registers and their fill level (wave-uniform)
+    double sx[kStage], sy[kStage];
+    int sqb[kStage];  // (bucket slot << kBinBits) | bin
+    int staged = 0;
+    // visits the staged candidates bin by bin, nearest bins first; stops at
+    // the first bin whose inner edge no lane's threshold reaches.  A
+    // candidate's bin b satisfies sqrt(lb2) >= b w (1 - 3e-7) (four float
+    // roundings), lb2 <= its squared distance as the test computes it
+    // (1 + 2^-51), so with the margins below no skipped candidate passes.
+    auto flush = [&]() {
+        for (int bn = 0; bn < kBins; bn++) {
+            const double edge = (double)bn * w * (1.0 - 1e-5);
+            if (__builtin_amdgcn_ballot_w64(edge * edge * (1.0 - 0x1p-40) <= kthr) == 0) break;
+#pragma unroll
+            for (int k = 0; k < kStage; k++) {
+                if (64 * k >= staged) break;
+                unsigned long long m =
+                    __builtin_amdgcn_ballot_w64(64 * k + lane < staged && (sqb[k] & ((1 << kBinBits) - 1)) == bn);
+                while (m) {
+                    const int l = (int)__builtin_ctzll(m);
+                    m &= m - 1;
+                    const double2 xy = make_double2(__shfl(sx[k], l, 64), __shfl(sy[k], l, 64));
+                    const int q = __builtin_amdgcn_readlane(sqb[k], l) >> kBinBits;
+                    const double dx = xy.x - cellX, dy = xy.y - cellY;
+                    visit(xy, __builtin_fma(dx, dx, dy * dy), q);
+                }
+            }
+        }
+        staged = 0;
+    };
+
+    for (int rad = 0;; rad++) {
+        const double searchRange = (double)(rad - 1) * p.block + minDistToBound;
+        if (act && p.buffer > 0 && searchRange >= p.buffer) {
+            act = false;
+            kthr = -INFINITY;
+        }
+        if (rad > p.maxRing) break;
+        if (__builtin_amdgcn_ballot_w64(act) == 0) break;
+        for (int tRow = blockRow - rad; tRow <= blockRow + rad; ++tRow) {
+            if (tRow < 0 || tRow >= p.blockRows) continue;
+            const bool full = (tRow == blockRow - rad) || (tRow == blockRow + rad);
+            const int step = full ? 1 : 2 * rad;
+            for (int tCol = blockCol - rad; tCol <= blockCol + rad; tCol += step) {
+                if (tCol < 0 || tCol >= p.blockCols) continue;
+                const int bi = tRow * p.blockCols + tCol;
+                const int s0 = p.start[bi], s1 = p.start[bi + 1];
+                if (act) ncand += (unsigned)p.count[bi];
+                if (s0 == s1) continue;
+                {
+                    const double4 bb = p.bbox[bi];
+                    const double dxb = fmax(fmax(bb.x - cellX, cellX - bb.y), 0.0);
+                    const double dyb = fmax(fmax(bb.z - cellY, cellY - bb.w), 0.0);
+                    const double lb2 = dxb * dxb + dyb * dyb;
+                    if (__builtin_amdgcn_ballot_w64(lb2 <= kthr) == 0) continue;
+                }
+                // stage the block's slots (padding slots hold NaN positions,
+                // which no test passes)
+                for (int q0 = s0; q0 < s1;) {
+                    const int take = min(s1 - q0, 64 * kStage - staged);
+#pragma unroll
+                    for (int k = 0; k < kStage; k++) {
+                        const int i = 64 * k + lane;
+                        if (i >= staged && i < staged + take) {
+                            const int q = q0 + (i - staged);
+                            const double2 xy = p.xy[q];
+                            const double ex = fmax(fmax(gx0 - xy.x, xy.x - gx1), 0.0);
+                            const double ey = fmax(fmax(gy0 - xy.y, xy.y - gy1), 0.0);
+                            const float r = __builtin_sqrtf((float)(ex * ex + ey * ey)) * inv_w;
+                            sx[k] = xy.x;
+                            sy[k] = xy.y;
+                            sqb[k] = (q << kBinBits) | (r < (float)(kBins - 1) ? (int)r : kBins - 1);  // NaN: last bin
+                        }
+                    }
+                    staged += take;
+                    q0 += take;
+                    if (staged == 64 * kStage) flush();
+                }
+            }
+        }
+        if (staged) flush();
+        if (act && tail >= K - 1 && searchRange >= maxDist) {
+            act = false;
+            kthr = -INFINITY;
+        }
+    }
+    tie = mine && tail >= K - 1 && runner == maxDist;
+    return (tail + 1 < K) ? tail + 1 : K;
+}
+
 // ADJOINT: instead of writing the grid, add g(cell) * w_i / W to the
 // gradient of each selected sample (bucket order).  The cells of a workgroup
 // select few distinct samples, so their contributions are first summed in
@@ -342,8 +533,13 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         atomicAdd(&p.grad_acc[q], v);
     };
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    const int col = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    // one workgroup per 16 x 16 tile, launched in row-major tile order or
+    // in the order of a load model (heaviest first)
+    const int tiles_x = (p.cols + 15) >> 4;
+    const int tile = p.order ? p.order[blockIdx.x] : (int)blockIdx.x;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int row = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int col = tx * 16 + (wave & 1) * 8 + (lane & 7);
     const bool inside = row < p.rows && col < p.cols;
     unsigned ncand = 0;
     float result = -9999.0f;
@@ -368,7 +564,15 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         const bool mine = pending && blockRow == gRow && blockCol == gCol;
         double *kd = kdist + threadIdx.x;
         int *ks = kslot + threadIdx.x;
-        const int num = idw_search_group(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand);
+        bool tie = false;
+        int num = idw_search_binned(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
+        // lanes whose list depends on the visiting order (an exact distance
+        // tie at the K-th place) search again in the reference's order
+        if (__builtin_amdgcn_ballot_w64(tie) != 0) {
+            unsigned again = 0;
+            const int num2 = idw_search_group(p, gRow, gCol, cellX, cellY, tie, kd, ks, again);
+            if (tie) num = num2;
+        }
         if (mine) {
             // weights in slot order (idwOperator.cpp:472-480)
             double wsum = 0.0;
@@ -394,10 +598,11 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
             if (q >= 0) atomicAdd(&p.grad_acc[q], tval[i]);
         }
     }
-    if (p.cand_out) {
+    if (p.cand_out || p.tile_work) {
         unsigned long long c = ncand;
         for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
-        if (lane == 0 && c) atomicAdd(p.cand_out, c);
+        if (lane == 0 && c && p.cand_out) atomicAdd(p.cand_out, c);
+        if (lane == 0 && c && p.tile_work) atomicAdd(&p.tile_work[tile], (double)c);
     }
 }
 
@@ -405,7 +610,7 @@ template <int MAXK, bool ADJOINT>
 static void launch_idw_k(const IdwParams &p, hipStream_t s)
 {
     dim3 block(256);
-    dim3 grid((p.cols + 15) / 16, (p.rows + 15) / 16);
+    dim3 grid((unsigned)(((p.cols + 15) / 16) * ((p.rows + 15) / 16)));
     hipLaunchKernelGGL((idw_kernel<MAXK, ADJOINT>), grid, block, 0, s, p);
 }
 
@@ -508,6 +713,8 @@ int idw_fill_params(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power
                     "searchRange <= 0 with %d samples < searchPointNum %d: the reference's search never ends",
                     plan->n, k);
     if (mask && (mask_rows < 1 || mask_cols < 1)) return fail(ctx, PARGO_E_INVALID, "bad mask dimensions");
+    if ((long long)((plan->rows + 15) / 16) * ((plan->cols + 15) / 16) > 0x7fffffffLL)
+        return fail(ctx, PARGO_E_INVALID, "grid of %d x %d cells has too many 16 x 16 tiles", plan->rows, plan->cols);
     p = IdwParams{};
     p.xy = plan->d_xy;
     p.v = plan->d_v;
@@ -530,6 +737,7 @@ int idw_fill_params(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power
     p.mask_rows = mask_rows;
     p.mask_cols = mask_cols;
     p.mask_nodata = mask_nodata;
+    p.order = plan->d_order;
     return PARGO_OK;
 }
 
@@ -558,8 +766,8 @@ int pargo_idw_extent(const double *xs, const double *ys, int n, float cellsize, 
     return PARGO_OK;
 }
 
-int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const double *vals, int n,
-                      float cellsize, double block_size, pargo_idw_plan **out)
+static int idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const double *vals, int n, float cellsize,
+                       double block_size, pargo_idw_plan **out)
 {
     if (!ctx) return PARGO_E_INVALID;
     if (!out || !xs || !ys || !vals || n < 1) return fail(ctx, PARGO_E_INVALID, "pargo_idw_prepare: bad arguments");
@@ -571,21 +779,20 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
     *out = nullptr;
     int rc = pargo::bind(ctx);
     if (rc) return rc;
-    pargo_idw_plan *pl = new (std::nothrow) pargo_idw_plan();
-    if (!pl) return fail(ctx, PARGO_E_NOMEM, "out of host memory");
+    pargo::alloc_point("IDW plan (host)");
+    // owned until it is handed out: every early return frees it
+    std::unique_ptr<pargo_idw_plan, PlanDeleter> pl(new pargo_idw_plan());
     pl->owner = ctx;
     pl->n = n;
     pl->cs = cellsize;
     pl->block = block_size;
     pargo::idw_extent(xs, ys, n, cellsize, pl->minX, pl->maxX, pl->minY, pl->maxY, pl->rows, pl->cols);
     if (pl->rows < 1 || pl->cols < 1) {
-        delete pl;
         return fail(ctx, PARGO_E_INVALID, "empty output grid");
     }
     // creatSampleBlocks: ceil of the extent over the block size
     const double br = std::ceil((pl->maxY - pl->minY) / block_size), bc = std::ceil((pl->maxX - pl->minX) / block_size);
     if (br * bc > 1 << 26) {
-        delete pl;
         return fail(ctx, PARGO_E_INVALID, "too many sample blocks (%.0f x %.0f)", br, bc);
     }
     pl->blockRows = (int)br;
@@ -597,8 +804,7 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
         const int iRow = (int)((pl->maxY - ys[i]) / block_size);
         const long long index = (long long)iRow * pl->blockCols + iCol;
         if (iCol < 0 || iRow < 0 || iCol >= pl->blockCols || index >= nb) {
-            delete pl;
-            return fail(ctx, PARGO_E_INVALID, "Index out of range. An error in creatSampleBlocks() (sample %d)", i);
+                return fail(ctx, PARGO_E_INVALID, "Index out of range. An error in creatSampleBlocks() (sample %d)", i);
         }
         bidx[i] = (int)index;
         start[index + 1]++;
@@ -609,6 +815,8 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
     start[0] = 0;
     for (int b = 0; b < nb; b++) start[b + 1] = start[b] + ((count[b] + 3) & ~3);
     const int slots = start[nb];
+    if (slots >= (1 << 25))  // the search packs a slot and a 6-bit bin in 31 bits
+        return fail(ctx, PARGO_E_UNSUPPORTED, "%d samples are more than the 2^25 bucket slots supported", n);
     pl->slots = slots;
     std::vector<int> fill(start.begin(), start.end() - 1);
     // 4 trailing padding slots: a lane may read one candidate past its bucket
@@ -634,11 +842,10 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
     const size_t bytes_bbox = sizeof(double4) * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t total = al(bytes_xy) + al(bytes_v) + al(bytes_start) + al(bytes_count) + al(bytes_perm) + al(bytes_bbox);
-    hipError_t e = hipMalloc(&pl->d_mem, total);
-    if (e != hipSuccess) {
-        delete pl;
-        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for IDW samples: %s", total, hipGetErrorString(e));
-    }
+    hipError_t e = pargo::dev_malloc(&pl->d_mem, total, "IDW samples");
+    if (e != hipSuccess)
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for IDW samples: %s%s", total, hipGetErrorString(e),
+                    pargo::injected_note());
     char *base = (char *)pl->d_mem;
     pl->d_xy = (double2 *)base;
     base += al(bytes_xy);
@@ -658,14 +865,18 @@ int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const 
     if (e == hipSuccess) e = hipMemcpyAsync(pl->d_perm, pl->h_perm.data(), bytes_perm, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(pl->d_bbox, bbox.data(), bytes_bbox, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        hipFree(pl->d_mem);
-        delete pl;
-        return fail(ctx, PARGO_E_DEVICE, "uploading IDW samples: %s", hipGetErrorString(e));
-    }
-    *out = pl;
+    if (e != hipSuccess) return fail(ctx, PARGO_E_DEVICE, "uploading IDW samples: %s", hipGetErrorString(e));
+    *out = pl.release();
     return PARGO_OK;
 }
+
+int pargo_idw_prepare(pargo_ctx *ctx, const double *xs, const double *ys, const double *vals, int n,
+                      float cellsize, double block_size, pargo_idw_plan **out)
+{
+    return pargo::guard(ctx, "pargo_idw_prepare",
+                        [&]() -> int { return idw_prepare(ctx, xs, ys, vals, n, cellsize, block_size, out); });
+}
+
 
 int pargo_idw_plan_free(pargo_ctx *ctx, pargo_idw_plan *plan)
 {
@@ -673,9 +884,7 @@ int pargo_idw_plan_free(pargo_ctx *ctx, pargo_idw_plan *plan)
     if (plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "IDW plan belongs to another context");
     int rc = pargo::bind(ctx);
     if (rc) return rc;
-    hipStreamSynchronize(ctx->stream);
-    if (plan->d_mem) hipFree(plan->d_mem);
-    delete plan;
+    PlanDeleter()(plan);
     return PARGO_OK;
 }
 
@@ -685,12 +894,16 @@ int pargo_idw_set_values(pargo_ctx *ctx, pargo_idw_plan *plan, const double *val
     if (!plan || !vals || plan->owner != ctx) return fail(ctx, PARGO_E_INVALID, "pargo_idw_set_values: bad arguments");
     int rc = pargo::bind(ctx);
     if (rc) return rc;
-    std::vector<double> v((size_t)plan->slots, 0.0);
-    for (int q = 0; q < plan->slots; q++)
-        if (plan->h_perm[q] >= 0) v[q] = vals[plan->h_perm[q]];
-    PARGO_HIP(ctx, hipMemcpyAsync(plan->d_v, v.data(), sizeof(double) * plan->slots, hipMemcpyHostToDevice, ctx->stream));
-    PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return PARGO_OK;
+    return pargo::guard(ctx, "pargo_idw_set_values", [&]() -> int {
+        pargo::alloc_point("IDW values (host)");
+        std::vector<double> v((size_t)plan->slots, 0.0);
+        for (int q = 0; q < plan->slots; q++)
+            if (plan->h_perm[q] >= 0) v[q] = vals[plan->h_perm[q]];
+        PARGO_HIP(ctx, hipMemcpyAsync(plan->d_v, v.data(), sizeof(double) * plan->slots, hipMemcpyHostToDevice,
+                                      ctx->stream));
+        PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return PARGO_OK;
+    });
 }
 
 int pargo_idw_set_values_device(pargo_ctx *ctx, pargo_idw_plan *plan, const double *d_vals)
@@ -789,6 +1002,85 @@ int pargo_idw_load(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int granul
                     "%u compute cells can never count searchPointNum %d samples: the reference's estimate never ends",
                     bad, k);
     return PARGO_OK;
+}
+
+static int idw_order_tiles(pargo_ctx *ctx, pargo_idw_plan *plan, int k, int power, double buffer,
+                           const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, int source)
+{
+    pargo::IdwParams p;
+    int rc = pargo::idw_fill_params(ctx, plan, k, power, buffer, mask, mask_rows, mask_cols, mask_nodata, p);
+    if (rc) return rc;
+    if (source != PARGO_ORDER_ROWS && source != PARGO_ORDER_ESTIMATE && source != PARGO_ORDER_MEASURED)
+        return fail(ctx, PARGO_E_INVALID, "unknown tile-order source %d", source);
+    rc = pargo::bind(ctx);
+    if (rc) return rc;
+    PARGO_HIP(ctx, hipStreamSynchronize(ctx->stream));  // runs queued on the old order have finished
+    if (plan->d_order) {
+        PARGO_HIP(ctx, hipFree(plan->d_order));
+        plan->d_order = nullptr;
+    }
+    plan->order_source = PARGO_ORDER_ROWS;
+    if (source == PARGO_ORDER_ROWS) return PARGO_OK;
+    const int tiles_x = (plan->cols + 15) / 16, tiles_y = (plan->rows + 15) / 16;
+    const size_t tiles = (size_t)tiles_x * tiles_y;
+    double *d_load = nullptr;
+    if (pargo::dev_malloc((void **)&d_load, tiles * sizeof(double), "tile loads") != hipSuccess)
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the tile loads%s", tiles * sizeof(double),
+                    pargo::injected_note());
+    // the load buffer is freed on every path, exceptions included
+    std::unique_ptr<double, decltype(&hipFree)> load_mem(d_load, &hipFree);
+    pargo::alloc_point("tile order (host)");
+    std::vector<double> load(tiles);
+    if (source == PARGO_ORDER_ESTIMATE) {
+        // the reference's estimate (IdwTransformation over a ComputeLayer of
+        // granularity 16): one coarse cell per tile
+        int lr = 0, lc = 0;
+        rc = pargo_idw_load(ctx, plan, k, 16, mask, mask_rows, mask_cols, (double)mask_nodata, -1.0, d_load, &lr, &lc);
+        if (rc == PARGO_OK && (lr != tiles_y || lc != tiles_x))
+            rc = fail(ctx, PARGO_E_INVALID, "load grid %d x %d does not match %d x %d tiles", lr, lc, tiles_y, tiles_x);
+    } else {
+        // the preliminary experiment: samples each tile's searches examined in one pass
+        hipError_t e = hipMemsetAsync(d_load, 0, tiles * sizeof(double), ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, PARGO_E_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
+        p.order = nullptr;
+        p.tile_work = d_load;
+        if (rc == PARGO_OK && pargo::launch_idw<false>(p, ctx->stream) != PARGO_OK)
+            rc = fail(ctx, PARGO_E_UNSUPPORTED, "unsupported searchPointNum %d", k);
+        if (rc == PARGO_OK) rc = pargo::check_launch(ctx, "pargo_idw_order_tiles");
+    }
+    if (rc == PARGO_OK) {
+        hipError_t e = hipMemcpyAsync(load.data(), d_load, tiles * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, PARGO_E_DEVICE, "reading the tile loads: %s", hipGetErrorString(e));
+    }
+    load_mem.reset();
+    if (rc) return rc;
+    // heaviest first; equal loads keep row-major order
+    std::vector<int> order(tiles);
+    for (size_t t = 0; t < tiles; t++) order[t] = (int)t;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return load[a] > load[b]; });
+    if (pargo::dev_malloc((void **)&plan->d_order, tiles * sizeof(int), "tile order") != hipSuccess) {
+        plan->d_order = nullptr;
+        return fail(ctx, PARGO_E_NOMEM, "hipMalloc(%zu) for the tile order%s", tiles * sizeof(int),
+                    pargo::injected_note());
+    }
+    hipError_t e = hipMemcpyAsync(plan->d_order, order.data(), tiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(plan->d_order);
+        plan->d_order = nullptr;
+        return fail(ctx, PARGO_E_DEVICE, "uploading the tile order: %s", hipGetErrorString(e));
+    }
+    plan->order_source = source;
+    return PARGO_OK;
+}
+
+int pargo_idw_order_tiles(pargo_ctx *ctx, pargo_idw_plan *plan, int k, int power, double buffer,
+                          const int32_t *mask, int mask_rows, int mask_cols, int mask_nodata, int source)
+{
+    return pargo::guard(ctx, "pargo_idw_order_tiles", [&]() -> int {
+        return idw_order_tiles(ctx, plan, k, power, buffer, mask, mask_rows, mask_cols, mask_nodata, source);
+    });
 }
 
 int pargo_idw_adjoint(pargo_ctx *ctx, const pargo_idw_plan *plan, int k, int power, double buffer,

@@ -7,8 +7,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/pargo.h"
+#include "faults.h"
 
 namespace pargo {
 struct BatchPlan;
@@ -37,6 +39,10 @@ struct pargo_ctx {
     int batch_cpt = 4;
     int batch_rows = 32;
     int batch_tuned = 0;
+    // queue order of batch tiles: PARGO_ORDER_ESTIMATE (cells), _ROWS (as
+    // given) or _MEASURED (batch_load, per-tile kernel times of one pass)
+    int batch_order = 1;
+    std::vector<double> batch_load;
 };
 
 namespace pargo {
@@ -55,6 +61,12 @@ int bind(pargo_ctx *ctx);
 
 // Checks the HIP error state after a launch.
 int check_launch(pargo_ctx *ctx, const char *what);
+
+// hipMalloc behind an allocation point (PARGO_FAIL_ALLOC, faults.h): a
+// selected point returns hipErrorOutOfMemory without allocating.
+hipError_t dev_malloc(void **ptr, size_t bytes, const char *site);
+// " (injected at <site>)" after an injected failure, else "" (clears it).
+const char *injected_note();
 
 // Grows the context scratch to at least `bytes`.
 int ensure_scratch(pargo_ctx *ctx, size_t bytes);

@@ -7,6 +7,7 @@
 #include <chrono>
 
 #include "../../include/pargo.h"
+#include "deviceRaster.h"
 
 namespace GPRO {
 
@@ -23,6 +24,7 @@ bool Application::END()
 {
     if (!_ctx) return true;
     const bool ok = deviceOk(pargo_sync(_ctx), "pargo_sync");
+    DeviceRaster::releaseAll();  // device copies of layers that outlive END
     pargo_destroy(_ctx);
     _ctx = nullptr;
     return ok;

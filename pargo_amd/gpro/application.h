@@ -20,6 +20,8 @@ class Application {
     // called).  Returns nullptr and prints the reason if the GPU library
     // cannot open a device.
     static pargo_ctx *device();
+    // The open context, or nullptr (never opens one).
+    static pargo_ctx *current() { return _ctx; }
     // Wall-clock seconds (the reference's MPI_Wtime).
     static double wtime();
 

@@ -19,6 +19,7 @@ struct DeviceBuffers {
     explicit DeviceBuffers(pargo_ctx *c) : ctx(c) {}
     ~DeviceBuffers()
     {
+        if (!ptrs.empty()) pargo_sync(ctx);
         for (void *p : ptrs) pargo_free(ctx, p);
     }
     void *alloc(size_t bytes, const char *what)
@@ -29,6 +30,24 @@ struct DeviceBuffers {
         return p;
     }
 };
+
+bool isInterior(const CoordBR &w, int rows, int cols)
+{
+    return w.minIRow() == 1 && w.minICol() == 1 && w.maxIRow() == rows - 2 && w.maxICol() == cols - 2;
+}
+
+// The outputs of one focal call: buffers, NoData mask and where they go.
+struct FocalOut {
+    float *slope = nullptr;
+    uint8_t *code = nullptr;
+    uint32_t *mask = nullptr;
+};
+
+bool runFocal(pargo_ctx *ctx, const pargo_grid &g, const float *dem, int method, int dirType, const FocalOut &o)
+{
+    return deviceOk(pargo_slope_d8(ctx, &g, dem, method, PARGO_UNIT_TAN, dirType, o.slope, o.code, o.mask),
+                    "pargo_slope_d8");
+}
 
 }  // namespace
 
@@ -52,14 +71,14 @@ bool focalOnDevice(const RasterLayer<double> &dem, const CoordBR &workBR, Raster
                    int slope_method, RasterLayer<double> *d8, int dir_type)
 {
     const MetaData *m = dem.metaData();
-    const CellSpace<double> *cs = dem.cellSpace();
-    if (!m || !cs || cs->empty()) {
+    if (!m || (!dem.hasCellSpace() && !dem.deviceCopy())) {
         cerr << "[ERROR] focal operator: the DEM layer holds no data" << endl;
         return false;
     }
-    const int rows = cs->nRows(), cols = cs->nCols();
+    const int rows = m->_localdims.nRows(), cols = m->_localdims.nCols();
     for (RasterLayer<double> *out : {slope, d8}) {
-        if (out && (!out->cellSpace() || out->cellSpace()->nRows() != rows || out->cellSpace()->nCols() != cols)) {
+        if (out && (!out->hasCellSpace() || !out->metaData() || out->metaData()->_localdims.nRows() != rows ||
+                    out->metaData()->_localdims.nCols() != cols)) {
             cerr << "[ERROR] focal operator: output layer " << out->name() << " does not match the DEM's "
                  << rows << " x " << cols << " cells (copyLayerInfo was not called)" << endl;
             return false;
@@ -69,56 +88,85 @@ bool focalOnDevice(const RasterLayer<double> &dem, const CoordBR &workBR, Raster
         cerr << "[ERROR] focal operator: work rectangle (" << workBR << ") outside the raster" << endl;
         return false;
     }
-    // float32 staging of the DEM, refusing values float32 would round
-    const size_t n = (size_t)rows * cols;
-    std::vector<float> host(n);
-    const double *src = cs->data();
-    for (size_t i = 0; i < n; i++) {
-        const float f = (float)src[i];
-        if ((double)f != src[i] && !(std::isnan(src[i]))) {
-            cerr << "[ERROR] focal operator: DEM value " << src[i] << " at row " << i / cols << ", column "
-                 << i % cols << " is not representable in float32, the device storage type of DEMs" << endl;
-            return false;
-        }
-        host[i] = f;
-    }
     pargo_ctx *ctx = Application::device();
     if (!ctx) return false;
-    DeviceBuffers buf(ctx);
-    const size_t mask_words = (size_t)rows * ((cols + 31) / 32);
-    float *d_dem = (float *)buf.alloc(n * sizeof(float), "pargo_alloc(dem)");
-    uint32_t *d_mask = (uint32_t *)buf.alloc(mask_words * 4, "pargo_alloc(nodata mask)");
-    float *d_slope = slope ? (float *)buf.alloc(n * sizeof(float), "pargo_alloc(slope)") : nullptr;
-    uint8_t *d_code = d8 ? (uint8_t *)buf.alloc(n, "pargo_alloc(d8)") : nullptr;
-    if (!d_dem || !d_mask || (slope && !d_slope) || (d8 && !d_code)) return false;
-
+    // the DEM as float32 in HBM: its current device copy, or uploaded once
+    // (and kept) from the host copy
+    const DeviceRaster *in = dem.deviceF32();
+    if (!in) return false;
     pargo_grid g;
     g.rows = rows;
     g.cols = cols;
     g.pitch = 0;
     g.cellsize = m->cellSize;
     g.nodata = m->noData;
-    if (!deviceOk(pargo_h2d(ctx, d_dem, host.data(), n * sizeof(float)), "pargo_h2d(dem)")) return false;
-    if (!deviceOk(pargo_slope_d8(ctx, &g, d_dem, slope_method, PARGO_UNIT_TAN, dir_type == 8 ? 8 : 256, d_slope,
-                                 d_code, d_mask),
-                  "pargo_slope_d8"))
-        return false;
-    std::vector<uint32_t> mask(mask_words);
+    const int dt = dir_type == 8 ? 8 : 256;
+
+    if (isInterior(workBR, rows, cols)) {
+        // Whole-layer outputs stay on the device: the kernels write NoData
+        // on the border, which is what copyLayerInfo left there.  Slope and
+        // D8 each get their own NoData mask (their NoData tests differ).
+        DeviceRaster *ds = slope ? slope->deviceOutput(DeviceForm::F32, true) : nullptr;
+        DeviceRaster *dc = d8 ? d8->deviceOutput(DeviceForm::Codes8, true) : nullptr;
+        if ((slope && !ds) || (d8 && !dc)) return false;
+        if (ds && dc) {
+            if (!deviceOk(pargo_slope(ctx, &g, (const float *)in->data, slope_method, PARGO_UNIT_TAN,
+                                      (float *)ds->data, ds->mask),
+                          "pargo_slope"))
+                return false;
+            if (!deviceOk(pargo_d8(ctx, &g, (const float *)in->data, dt, (uint8_t *)dc->data, dc->mask), "pargo_d8"))
+                return false;
+        } else {
+            FocalOut o;
+            o.slope = ds ? (float *)ds->data : nullptr;
+            o.code = dc ? (uint8_t *)dc->data : nullptr;
+            o.mask = ds ? ds->mask : dc->mask;
+            if (!runFocal(ctx, g, (const float *)in->data, slope_method, dt, o)) return false;
+        }
+        if (dc) dc->dirType = dt;
+        // results ready when the operator returns (its run time includes
+        // them, and a failure is reported here)
+        if (!deviceOk(pargo_sync(ctx), "pargo_sync")) return false;
+        if (slope) slope->deviceWritten();
+        if (d8) d8->deviceWritten();
+        return true;
+    }
+
+    // Another work rectangle: compute the whole raster, then write the
+    // rectangle's cells into the host layers (the others keep their values).
+    DeviceBuffers buf(ctx);
+    const size_t n = (size_t)rows * cols, wpr = (size_t)(cols + 31) / 32;
+    FocalOut o;
+    o.mask = (uint32_t *)buf.alloc((size_t)rows * wpr * 4, "pargo_alloc(nodata mask)");
+    o.slope = slope ? (float *)buf.alloc(n * sizeof(float), "pargo_alloc(slope)") : nullptr;
+    o.code = d8 ? (uint8_t *)buf.alloc(n, "pargo_alloc(d8)") : nullptr;
+    if (!o.mask || (slope && !o.slope) || (d8 && !o.code)) return false;
+    std::vector<uint32_t> mask((size_t)rows * wpr);
     std::vector<float> hs(slope ? n : 0);
     std::vector<uint8_t> hc(d8 ? n : 0);
-    if (!deviceOk(pargo_d2h(ctx, mask.data(), d_mask, mask_words * 4), "pargo_d2h(mask)")) return false;
-    if (slope && !deviceOk(pargo_d2h(ctx, hs.data(), d_slope, n * sizeof(float)), "pargo_d2h(slope)")) return false;
-    if (d8 && !deviceOk(pargo_d2h(ctx, hc.data(), d_code, n), "pargo_d2h(d8)")) return false;
-    if (!deviceOk(pargo_sync(ctx), "pargo_sync")) return false;
-
-    const size_t wpr = (size_t)(cols + 31) / 32;
-    const uint8_t flat = dir_type == 8 ? PARGO_D8_FLAT_8 : PARGO_D8_FLAT_256;
-    for (int r = workBR.minIRow(); r <= workBR.maxIRow(); r++) {
-        for (int c = workBR.minICol(); c <= workBR.maxICol(); c++) {
-            const size_t i = (size_t)r * cols + c;
-            const bool nd = (mask[r * wpr + c / 32] >> (c % 32)) & 1u;
-            if (slope) (*slope->cellSpace())[r][c] = nd ? slope->metaData()->noData : (double)hs[i];
-            if (d8) (*d8->cellSpace())[r][c] = nd ? d8->metaData()->noData : (hc[i] == flat ? -1.0 : (double)hc[i]);
+    // one output per call when both are wanted: their NoData masks differ
+    for (RasterLayer<double> *out : {slope, d8}) {
+        if (!out) continue;
+        const bool is_s = out == slope;
+        FocalOut q;
+        q.slope = is_s ? o.slope : nullptr;
+        q.code = is_s ? nullptr : o.code;
+        q.mask = o.mask;
+        if (!runFocal(ctx, g, (const float *)in->data, slope_method, dt, q)) return false;
+        if (!deviceOk(pargo_d2h(ctx, mask.data(), o.mask, mask.size() * 4), "pargo_d2h(mask)")) return false;
+        if (is_s && !deviceOk(pargo_d2h(ctx, hs.data(), o.slope, n * sizeof(float)), "pargo_d2h(slope)")) return false;
+        if (!is_s && !deviceOk(pargo_d2h(ctx, hc.data(), o.code, n), "pargo_d2h(d8)")) return false;
+        DeviceTransfers::d2h += mask.size() * 4 + (is_s ? n * sizeof(float) : n);
+        CellSpace<double> &cs = *out->cellSpace();
+        const double nd = out->metaData()->noData;
+        const uint8_t flat = dt == 8 ? PARGO_D8_FLAT_8 : PARGO_D8_FLAT_256;
+#pragma omp parallel for schedule(static)
+        for (int r = workBR.minIRow(); r <= workBR.maxIRow(); r++) {
+            for (int c = workBR.minICol(); c <= workBR.maxICol(); c++) {
+                const size_t i = (size_t)r * cols + c;
+                const bool isnd = (mask[r * wpr + c / 32] >> (c % 32)) & 1u;
+                cs[r][c] = isnd ? nd : (is_s ? (double)hs[i] : (hc[i] == flat ? -1.0 : (double)hc[i]));
+            }
         }
     }
     return true;

@@ -9,6 +9,8 @@
 #include <iterator>
 #include <map>
 
+#include "../csrc/faults.h"
+
 namespace GPRO {
 namespace geotiff {
 
@@ -70,10 +72,21 @@ int type_size(uint16_t t)
     }
 }
 
+// count * size of an entry's values, or false when it cannot fit the file
+// (the product would overflow only for counts no file can hold)
+bool entry_bytes(const File &f, const Entry &e, uint64_t &bytes)
+{
+    const int sz = type_size(e.type);
+    if (!sz || e.count > f.b.size()) return false;
+    bytes = e.count * (uint64_t)sz;
+    return f.ok(e.value_off, bytes);
+}
+
 bool values(const File &f, const Entry &e, std::vector<double> &out)
 {
     const int sz = type_size(e.type);
-    if (!sz || !f.ok(e.value_off, e.count * (uint64_t)sz)) return false;
+    uint64_t bytes = 0;
+    if (!entry_bytes(f, e, bytes)) return false;
     out.resize(e.count);
     for (uint64_t i = 0; i < e.count; i++) {
         const uint64_t o = e.value_off + i * sz;
@@ -99,7 +112,8 @@ bool values(const File &f, const Entry &e, std::vector<double> &out)
 bool uvalues(const File &f, const Entry &e, std::vector<uint64_t> &out)
 {
     const int sz = type_size(e.type);
-    if (!sz || !f.ok(e.value_off, e.count * (uint64_t)sz)) return false;
+    uint64_t bytes = 0;
+    if (!entry_bytes(f, e, bytes)) return false;
     out.resize(e.count);
     for (uint64_t i = 0; i < e.count; i++) out[i] = f.u(e.value_off + i * sz, sz);
     return true;
@@ -212,6 +226,7 @@ bool parse(const std::string &path, File &f, std::map<uint16_t, Entry> &tags, st
         err = "cannot open " + path;
         return false;
     }
+    pargo::alloc_point("geotiff.file");
     f.b.assign(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>());
     if (f.b.size() < 8) {
         err = path + ": not a TIFF file";
@@ -241,7 +256,7 @@ bool parse(const std::string &path, File &f, std::map<uint16_t, Entry> &tags, st
         return false;
     }
     const uint64_t n = f.u(ifd, cnt_sz);
-    if (!f.ok(ifd + cnt_sz, n * ent_sz)) {
+    if (n > f.b.size() / ent_sz || !f.ok(ifd + cnt_sz, n * ent_sz)) {
         err = path + ": truncated IFD";
         return false;
     }
@@ -252,8 +267,9 @@ bool parse(const std::string &path, File &f, std::map<uint16_t, Entry> &tags, st
         e.type = (uint16_t)f.u(o + 2, 2);
         e.count = f.u(o + 4, f.big ? 8 : 4);
         const uint64_t vo = o + (f.big ? 12 : 8);
-        const uint64_t bytes = e.count * (uint64_t)type_size(e.type);
-        e.value_off = bytes <= (uint64_t)val_sz ? vo : f.u(vo, val_sz);
+        // values that fit the entry's value field sit there, others at an offset
+        const bool inline_vals = e.count <= (uint64_t)val_sz && e.count * (uint64_t)type_size(e.type) <= (uint64_t)val_sz;
+        e.value_off = inline_vals ? vo : f.u(vo, val_sz);
         tags[tag] = e;
     }
     return true;
@@ -268,17 +284,33 @@ double get1(const File &f, const std::map<uint16_t, Entry> &tags, uint16_t tag, 
     return v[0];
 }
 
+// An integer tag within [lo, hi]; false when it is not (a fractional, huge
+// or non-finite value never reaches an int conversion).
+bool get_int(const File &f, const std::map<uint16_t, Entry> &tags, uint16_t tag, long long dflt, long long lo,
+             long long hi, long long &out)
+{
+    const double v = get1(f, tags, tag, (double)dflt);
+    if (!(v >= (double)lo && v <= (double)hi) || v != std::floor(v)) return false;
+    out = (long long)v;
+    return true;
+}
+
 bool fill_info(const File &f, const std::map<uint16_t, Entry> &tags, Info &info, std::string &err,
                const std::string &path)
 {
-    info.cols = (int)get1(f, tags, T_WIDTH, 0);
-    info.rows = (int)get1(f, tags, T_LENGTH, 0);
-    info.bits = (int)get1(f, tags, T_BITS, 1);
-    info.sample_format = (int)get1(f, tags, T_SAMPLE_FORMAT, 1);
-    if (info.rows <= 0 || info.cols <= 0) {
-        err = path + ": missing image dimensions";
+    long long cols = 0, rows = 0, bits = 1, fmt = 1;
+    if (!get_int(f, tags, T_WIDTH, 0, 1, INT32_MAX, cols) || !get_int(f, tags, T_LENGTH, 0, 1, INT32_MAX, rows)) {
+        err = path + ": missing or bad image dimensions";
         return false;
     }
+    if (!get_int(f, tags, T_BITS, 1, 1, 64, bits) || !get_int(f, tags, T_SAMPLE_FORMAT, 1, 1, 6, fmt)) {
+        err = path + ": bad sample description";
+        return false;
+    }
+    info.cols = (int)cols;
+    info.rows = (int)rows;
+    info.bits = (int)bits;
+    info.sample_format = (int)fmt;
     std::vector<double> scale, tie, m;
     auto it = tags.find(T_PIXEL_SCALE);
     if (it != tags.end()) values(f, it->second, scale);
@@ -339,10 +371,12 @@ bool read(const std::string &path, Info &info, std::vector<double> &data, std::s
     std::map<uint16_t, Entry> tags;
     if (!parse(path, f, tags, err)) return false;
     if (!fill_info(f, tags, info, err, path)) return false;
-    const int spp = (int)get1(f, tags, T_SPP, 1);
-    const int planar = (int)get1(f, tags, T_PLANAR, 1);
-    const int comp = (int)get1(f, tags, T_COMPRESSION, 1);
-    const int pred = (int)get1(f, tags, T_PREDICTOR, 1);
+    long long spp = 1, planar = 1, comp = 1, pred = 1;
+    if (!get_int(f, tags, T_SPP, 1, 1, 65535, spp) || !get_int(f, tags, T_PLANAR, 1, 1, 2, planar) ||
+        !get_int(f, tags, T_COMPRESSION, 1, 1, 65535, comp) || !get_int(f, tags, T_PREDICTOR, 1, 1, 3, pred)) {
+        err = path + ": bad sample layout, compression or predictor tag";
+        return false;
+    }
     const int bps = info.bits / 8;
     const int fmt = info.sample_format;
     if (!(bps == 1 || bps == 2 || bps == 4 || bps == 8) || info.bits % 8) {
@@ -354,9 +388,17 @@ bool read(const std::string &path, Info &info, std::vector<double> &data, std::s
         return false;
     }
     const bool tiled = tags.count(T_TILE_OFFSETS) != 0;
-    const int bw = tiled ? (int)get1(f, tags, T_TILE_W, 0) : info.cols;
-    const int bh = tiled ? (int)get1(f, tags, T_TILE_H, 0) : (int)std::min<double>(get1(f, tags, T_ROWS_PER_STRIP, info.rows), info.rows);
-    if (bw <= 0 || bh <= 0) {
+    long long bw = info.cols, bh = info.rows;
+    bool block_ok = true;
+    if (tiled) {
+        block_ok = get_int(f, tags, T_TILE_W, 0, 1, INT32_MAX, bw) && get_int(f, tags, T_TILE_H, 0, 1, INT32_MAX, bh);
+    } else {
+        // RowsPerStrip may be 2^32-1 ("one strip"); only its minimum with the rows matters
+        const double rps = get1(f, tags, T_ROWS_PER_STRIP, info.rows);
+        block_ok = rps >= 1 && rps == std::floor(rps);
+        if (block_ok) bh = (long long)std::min<double>(rps, info.rows);
+    }
+    if (!block_ok) {
         err = path + ": bad block size";
         return false;
     }
@@ -366,24 +408,47 @@ bool read(const std::string &path, Info &info, std::vector<double> &data, std::s
         err = path + ": bad strip/tile tables";
         return false;
     }
-    const int sample_stride = planar == 1 ? spp : 1;  // band 1 only
-    const int across = (info.cols + bw - 1) / bw, down = (info.rows + bh - 1) / bh;
-    if (offs.size() < (size_t)across * down) {
+    const int sample_stride = planar == 1 ? (int)spp : 1;  // band 1 only
+    const long long across = (info.cols + bw - 1) / bw, down = (info.rows + bh - 1) / bh;
+    if ((uint64_t)across * (uint64_t)down > offs.size()) {
         err = path + ": too few strips/tiles";
         return false;
     }
+    // Sizes the file cannot back are refused before anything is allocated:
+    // a block decodes to at most 3641 bytes per compressed byte (a 9-bit LZW
+    // code emits at most 4096 bytes; PackBits at most 64 per byte), the
+    // image to at most that over the whole file, and an uncompressed block
+    // must be present in full.  Tiles may overhang the image (a 10 x 10
+    // image in one 256 x 256 tile), so a block may be 64 MiB even then.
+    const uint64_t expand = comp == 1 ? 1 : 3641;
+    const uint64_t image_bytes = (uint64_t)info.rows * (uint64_t)info.cols * (uint64_t)bps * (uint64_t)sample_stride;
+    const uint64_t block_px = (uint64_t)bw * (uint64_t)bh;
+    if (image_bytes / expand > f.b.size() || block_px > (1ull << 40) ||
+        block_px * (uint64_t)bps * (uint64_t)sample_stride > std::max<uint64_t>(image_bytes, 1ull << 26)) {
+        err = path + ": the image or block size exceeds what the file holds";
+        return false;
+    }
+    pargo::alloc_point("geotiff.raster");
     data.assign((size_t)info.rows * info.cols, 0.0);
+    pargo::alloc_point("geotiff.block");
     std::vector<unsigned char> buf;
-    const size_t block_bytes = (size_t)bw * bh * bps * sample_stride;
-    for (int by = 0; by < down; by++)
-        for (int bx = 0; bx < across; bx++) {
+    const size_t block_bytes = (size_t)block_px * bps * sample_stride;
+    for (long long by = 0; by < down; by++)
+        for (long long bx = 0; bx < across; bx++) {
             const size_t k = (size_t)by * across + bx;
             if (!f.ok(offs[k], counts[k])) {
                 err = path + ": truncated data";
                 return false;
             }
-            const unsigned char *src = &f.b[offs[k]];
+            const unsigned char *src = f.b.data() + offs[k];
             if (comp == 1) {
+                // rows of the block inside the image must be present
+                const uint64_t need = (uint64_t)std::min<long long>(bh, info.rows - (long long)by * bh) * bw * bps *
+                                      sample_stride;
+                if (counts[k] < need) {
+                    err = path + ": truncated strip or tile";
+                    return false;
+                }
                 buf.assign(src, src + std::min<uint64_t>(counts[k], block_bytes));
             } else if (comp == 5) {
                 if (!lzw_decode(src, counts[k], buf, block_bytes)) {
@@ -399,7 +464,7 @@ bool read(const std::string &path, Info &info, std::vector<double> &data, std::s
             if (pred == 2) {  // horizontal differencing, per sample
                 for (int y = 0; y < bh; y++) {
                     unsigned char *rb = buf.data() + y * row_bytes;
-                    for (int x = sample_stride; x < bw * sample_stride; x++) {
+                    for (long long x = sample_stride; x < bw * sample_stride; x++) {
                         const size_t a = (size_t)x * bps, p = (size_t)(x - sample_stride) * bps;
                         uint64_t cur = 0, prev = 0;
                         for (int q = 0; q < bps; q++) {
@@ -427,10 +492,10 @@ bool read(const std::string &path, Info &info, std::vector<double> &data, std::s
                 }
             }
             for (int y = 0; y < bh; y++) {
-                const int r = by * bh + y;
+                const long long r = by * bh + y;
                 if (r >= info.rows) break;
                 for (int x = 0; x < bw; x++) {
-                    const int c = bx * bw + x;
+                    const long long c = bx * bw + x;
                     if (c >= info.cols) break;
                     const size_t o = ((size_t)y * bw + x) * bps * sample_stride;
                     uint64_t raw = 0;
@@ -489,6 +554,7 @@ bool write(const std::string &path, const Info &info, const double *data, Type t
         err = "cannot create " + path;
         return false;
     }
+    pargo::alloc_point("geotiff.header");
     Writer w;
     w.b.reserve(1024);
     w.b.push_back('I'), w.b.push_back('I');
@@ -593,6 +659,7 @@ bool write(const std::string &path, const Info &info, const double *data, Type t
         }
     }
     os.write((const char *)w.b.data(), (std::streamsize)w.b.size());
+    pargo::alloc_point("geotiff.row");
     std::vector<unsigned char> row((size_t)cols * bps);
     for (uint32_t r = 0; r < rows; r++) {
         const double *src = data + (size_t)r * cols;

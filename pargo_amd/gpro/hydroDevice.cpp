@@ -18,6 +18,7 @@ struct DeviceBuffers {
     explicit DeviceBuffers(pargo_ctx *c) : ctx(c) {}
     ~DeviceBuffers()
     {
+        if (!ptrs.empty()) pargo_sync(ctx);
         for (void *p : ptrs) pargo_free(ctx, p);
     }
     void *alloc(size_t bytes, const char *what)
@@ -29,18 +30,40 @@ struct DeviceBuffers {
     }
 };
 
+bool isInteriorRect(const CoordBR &w, int rows, int cols)
+{
+    return w.minIRow() == 1 && w.minICol() == 1 && w.maxIRow() == rows - 2 && w.maxICol() == cols - 2;
+}
+
+// Writes the work rectangle of a device float64 result into the host layer
+// (the other cells keep their values), or, for the interior rectangle, keeps
+// the whole result on the device as the layer's copy.
+bool deliverF64(pargo_ctx *ctx, const double *d_out, const CoordBR &workBR, int rows, int cols,
+                RasterLayer<double> &layer)
+{
+    const size_t n = (size_t)rows * cols;
+    std::vector<double> out(n);
+    if (!deviceOk(pargo_d2h(ctx, out.data(), d_out, n * sizeof(double)), "pargo_d2h(result)")) return false;
+    DeviceTransfers::d2h += n * sizeof(double);
+    CellSpace<double> &cs = *layer.cellSpace();
+#pragma omp parallel for schedule(static)
+    for (int r = workBR.minIRow(); r <= workBR.maxIRow(); r++)
+        for (int c = workBR.minICol(); c <= workBR.maxICol(); c++) cs[r][c] = out[(size_t)r * cols + c];
+    return true;
+}
+
 }  // namespace
 
 bool scaOnDevice(const RasterLayer<double> &d8, const CoordBR &workBR, RasterLayer<double> &sca, int mode)
 {
     const MetaData *m = d8.metaData();
-    const CellSpace<double> *cs = d8.cellSpace();
-    if (!m || !cs || cs->empty()) {
+    if (!m || (!d8.hasCellSpace() && !d8.deviceCopy())) {
         cerr << "[ERROR] SCA operator: the D8 layer holds no data" << endl;
         return false;
     }
-    const int rows = cs->nRows(), cols = cs->nCols();
-    if (!sca.cellSpace() || sca.cellSpace()->nRows() != rows || sca.cellSpace()->nCols() != cols) {
+    const int rows = m->_localdims.nRows(), cols = m->_localdims.nCols();
+    if (!sca.hasCellSpace() || !sca.metaData() || sca.metaData()->_localdims.nRows() != rows ||
+        sca.metaData()->_localdims.nCols() != cols) {
         cerr << "[ERROR] SCA operator: output layer " << sca.name() << " does not match the D8 layer's " << rows
              << " x " << cols << " cells (copyLayerInfo was not called)" << endl;
         return false;
@@ -53,37 +76,56 @@ bool scaOnDevice(const RasterLayer<double> &d8, const CoordBR &workBR, RasterLay
     if (!ctx) return false;
     DeviceBuffers buf(ctx);
     const size_t n = (size_t)rows * cols;
-    double *d_d8 = (double *)buf.alloc(n * sizeof(double), "pargo_alloc(d8)");
-    uint8_t *d_flow = (uint8_t *)buf.alloc(n, "pargo_alloc(flow classes)");
-    double *d_sca = (double *)buf.alloc(n * sizeof(double), "pargo_alloc(sca)");
-    if (!d_d8 || !d_flow || !d_sca) return false;
     pargo_grid g;
     g.rows = rows;
     g.cols = cols;
     g.pitch = 0;
     g.cellsize = m->cellSize;
     g.nodata = m->noData;
-    std::vector<double> out(n);
-    if (!deviceOk(pargo_h2d(ctx, d_d8, cs->data(), n * sizeof(double)), "pargo_h2d(d8)")) return false;
-    if (!deviceOk(pargo_d8_classify(ctx, &g, d_d8, d_flow), "pargo_d8_classify")) return false;
-    if (!deviceOk(pargo_sca_d8(ctx, &g, d_flow, nullptr, mode, d_sca), "pargo_sca_d8")) return false;
-    if (!deviceOk(pargo_d2h(ctx, out.data(), d_sca, n * sizeof(double)), "pargo_d2h(sca)")) return false;
-    if (!deviceOk(pargo_sync(ctx), "pargo_sync")) return false;
-    for (int r = workBR.minIRow(); r <= workBR.maxIRow(); r++)
-        for (int c = workBR.minICol(); c <= workBR.maxICol(); c++) (*sca.cellSpace())[r][c] = out[(size_t)r * cols + c];
-    return true;
+    // Flow classes: a D8 layer left on the device by the D8 operator with
+    // dirType 8 already holds them (window index, 255 = flat, NoData mask);
+    // otherwise the layer's float64 codes are classified on the device with
+    // the reference's comparisons.
+    const uint8_t *flow = nullptr;
+    const uint32_t *mask = nullptr;
+    const DeviceRaster *dd = d8.deviceCopy();
+    if (dd && dd->form == DeviceForm::Codes8 && dd->dirType == 8) {
+        flow = (const uint8_t *)dd->data;
+        mask = dd->mask;
+    } else {
+        double *d_d8 = (double *)buf.alloc(n * sizeof(double), "pargo_alloc(d8)");
+        uint8_t *d_flow = (uint8_t *)buf.alloc(n, "pargo_alloc(flow classes)");
+        if (!d_d8 || !d_flow) return false;
+        if (!staging::uploadF64(d8.cellSpace()->data(), n, d_d8)) return false;
+        if (!deviceOk(pargo_d8_classify(ctx, &g, d_d8, d_flow), "pargo_d8_classify")) return false;
+        flow = d_flow;
+    }
+    if (isInteriorRect(workBR, rows, cols)) {
+        // the kernels write the reference's NoData on the border and on
+        // NoData cells: the whole layer stays on the device
+        DeviceRaster *out = sca.deviceOutput(DeviceForm::F64, false);
+        if (!out) return false;
+        if (!deviceOk(pargo_sca_d8(ctx, &g, flow, mask, mode, (double *)out->data), "pargo_sca_d8")) return false;
+        if (!deviceOk(pargo_sync(ctx), "pargo_sync")) return false;
+        sca.deviceWritten();
+        return true;
+    }
+    double *d_sca = (double *)buf.alloc(n * sizeof(double), "pargo_alloc(sca)");
+    if (!d_sca) return false;
+    if (!deviceOk(pargo_sca_d8(ctx, &g, flow, mask, mode, d_sca), "pargo_sca_d8")) return false;
+    return deliverF64(ctx, d_sca, workBR, rows, cols, sca);
 }
 
 bool fillOnDevice(const RasterLayer<double> &dem, const CoordBR &workBR, RasterLayer<double> &wdem)
 {
     const MetaData *m = dem.metaData();
-    const CellSpace<double> *cs = dem.cellSpace();
-    if (!m || !cs || cs->empty()) {
+    if (!m || (!dem.hasCellSpace() && !dem.deviceCopy())) {
         cerr << "[ERROR] pit removal: the DEM layer holds no data" << endl;
         return false;
     }
-    const int rows = cs->nRows(), cols = cs->nCols();
-    if (!wdem.cellSpace() || wdem.cellSpace()->nRows() != rows || wdem.cellSpace()->nCols() != cols) {
+    const int rows = m->_localdims.nRows(), cols = m->_localdims.nCols();
+    if (!wdem.hasCellSpace() || !wdem.metaData() || wdem.metaData()->_localdims.nRows() != rows ||
+        wdem.metaData()->_localdims.nCols() != cols) {
         cerr << "[ERROR] pit removal: output layer " << wdem.name() << " does not match the DEM's " << rows << " x "
              << cols << " cells (copyLayerInfo was not called)" << endl;
         return false;
@@ -92,39 +134,33 @@ bool fillOnDevice(const RasterLayer<double> &dem, const CoordBR &workBR, RasterL
         cerr << "[ERROR] pit removal: work rectangle (" << workBR << ") outside the raster" << endl;
         return false;
     }
-    const size_t n = (size_t)rows * cols;
-    std::vector<float> host(n);
-    const double *src = cs->data();
-    for (size_t i = 0; i < n; i++) {
-        const float f = (float)src[i];
-        if ((double)f != src[i] && !std::isnan(src[i])) {
-            cerr << "[ERROR] pit removal: DEM value " << src[i] << " at row " << i / cols << ", column " << i % cols
-                 << " is not representable in float32, the device storage type of DEMs" << endl;
-            return false;
-        }
-        host[i] = f;
-    }
     pargo_ctx *ctx = Application::device();
     if (!ctx) return false;
-    DeviceBuffers buf(ctx);
-    float *d_dem = (float *)buf.alloc(n * sizeof(float), "pargo_alloc(dem)");
-    double *d_w = (double *)buf.alloc(n * sizeof(double), "pargo_alloc(filled)");
-    if (!d_dem || !d_w) return false;
+    const DeviceRaster *in = dem.deviceF32();  // the DEM's device copy, uploaded once
+    if (!in) return false;
     pargo_grid g;
     g.rows = rows;
     g.cols = cols;
     g.pitch = 0;
     g.cellsize = m->cellSize;
     g.nodata = m->noData;
-    std::vector<double> out(n);
     int passes = 0;
-    if (!deviceOk(pargo_h2d(ctx, d_dem, host.data(), n * sizeof(float)), "pargo_h2d(dem)")) return false;
-    if (!deviceOk(pargo_pit_remove(ctx, &g, d_dem, d_w, &passes), "pargo_pit_remove")) return false;
-    if (!deviceOk(pargo_d2h(ctx, out.data(), d_w, n * sizeof(double)), "pargo_d2h(filled)")) return false;
-    if (!deviceOk(pargo_sync(ctx), "pargo_sync")) return false;
-    for (int r = workBR.minIRow(); r <= workBR.maxIRow(); r++)
-        for (int c = workBR.minICol(); c <= workBR.maxICol(); c++) (*wdem.cellSpace())[r][c] = out[(size_t)r * cols + c];
-    return true;
+    if (isInteriorRect(workBR, rows, cols)) {
+        // NoData on the border, as copyLayerInfo left it: the whole layer
+        // stays on the device (pargo_pit_remove returns when it is done)
+        DeviceRaster *out = wdem.deviceOutput(DeviceForm::F64, false);
+        if (!out) return false;
+        if (!deviceOk(pargo_pit_remove(ctx, &g, (const float *)in->data, (double *)out->data, &passes),
+                      "pargo_pit_remove"))
+            return false;
+        wdem.deviceWritten();
+        return true;
+    }
+    DeviceBuffers buf(ctx);
+    double *d_w = (double *)buf.alloc((size_t)rows * cols * sizeof(double), "pargo_alloc(filled)");
+    if (!d_w) return false;
+    if (!deviceOk(pargo_pit_remove(ctx, &g, (const float *)in->data, d_w, &passes), "pargo_pit_remove")) return false;
+    return deliverF64(ctx, d_w, workBR, rows, cols, wdem);
 }
 
 }  // namespace GPRO

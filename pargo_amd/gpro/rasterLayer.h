@@ -1,7 +1,12 @@
 // rasterLayer.h -- raster layer of the gpro interface (gpro/rasterLayer.h of
 // the reference: RasterLayer<elemType> :41-236).
 //
-// A layer couples a host CellSpace, an optional Neighborhood and a MetaData.
+// A layer couples a host CellSpace, an optional Neighborhood and a MetaData,
+// and (double layers) a device-resident copy of its cells (deviceRaster.h):
+// device operators read and leave their data there, and the host CellSpace
+// is brought up to date only when host code asks for it through
+// cellSpace() or writes the file.  Non-const cellSpace() access counts as a
+// host write and retires the device copy.
 // Files are GeoTIFFs read and written by the GDAL-free codec in geotiff.h.
 // One process owns the whole raster (this build runs one process per GPU),
 // so every decomposition type reads all rows; the work rectangle is the set
@@ -11,8 +16,10 @@
 #include <fstream>
 #include <memory>
 #include <string>
+#include <type_traits>
 
 #include "cellSpace.h"
+#include "deviceRaster.h"
 #include "geotiff.h"
 #include "metaData.h"
 #include "neighborhood.h"
@@ -30,6 +37,8 @@ class RasterLayer {
         if (this == &rhs) return *this;
         _strLayerName = rhs._strLayerName;
         _nLayerID = rhs._nLayerID;
+        rhs.syncHost();
+        _dev.reset();
         _pCellSpace.reset(rhs._pCellSpace ? new CellSpace<elemType>(*rhs._pCellSpace) : nullptr);
         _pNbrhood.reset(rhs._pNbrhood ? new Neighborhood<elemType>(*rhs._pNbrhood) : nullptr);
         _meta.reset(rhs._meta ? new MetaData(*rhs._meta) : nullptr);
@@ -43,25 +52,34 @@ class RasterLayer {
     int id() const { return _nLayerID; }
     void id(int i) { _nLayerID = i; }
 
-    CellSpace<elemType> *cellSpace() { return _pCellSpace.get(); }
-    const CellSpace<elemType> *cellSpace() const { return _pCellSpace.get(); }
+    CellSpace<elemType> *cellSpace()
+    {
+        syncHost();
+        retireDevice();  // the caller may write the cells
+        return _pCellSpace.get();
+    }
+    const CellSpace<elemType> *cellSpace() const
+    {
+        syncHost();
+        return _pCellSpace.get();
+    }
     Neighborhood<elemType> *nbrhood() { return _pNbrhood.get(); }
     const Neighborhood<elemType> *nbrhood() const { return _pNbrhood.get(); }
     MetaData *metaData() { return _pMetaData; }
     const MetaData *metaData() const { return _pMetaData; }
 
-    void cleanCellSpace() { _pCellSpace.reset(); }
+    void cleanCellSpace() { _pCellSpace.reset(), _dev.reset(); }
     void cleanNbrhood() { _pNbrhood.reset(); }
     void cleanMetaData() { _meta.reset(), _pMetaData = nullptr; }
     bool hasCellSpace() const { return (bool)_pCellSpace; }
     bool hasNbrhood() const { return (bool)_pNbrhood; }
     bool hasMetaData() const { return _pMetaData != nullptr; }
 
-    bool newCellSpace() { return _pCellSpace.reset(new CellSpace<elemType>()), true; }
-    bool newCellSpace(const SpaceDims &d) { return _pCellSpace.reset(new CellSpace<elemType>(d)), true; }
+    bool newCellSpace() { return retireDevice(), _pCellSpace.reset(new CellSpace<elemType>()), true; }
+    bool newCellSpace(const SpaceDims &d) { return retireDevice(), _pCellSpace.reset(new CellSpace<elemType>(d)), true; }
     bool newCellSpace(const SpaceDims &d, const elemType &v)
     {
-        return _pCellSpace.reset(new CellSpace<elemType>(d, v)), true;
+        return retireDevice(), _pCellSpace.reset(new CellSpace<elemType>(d, v)), true;
     }
     bool newCellSpace(int r, int c) { return newCellSpace(SpaceDims(r, c)); }
     bool newCellSpace(int r, int c, const elemType &v) { return newCellSpace(SpaceDims(r, c), v); }
@@ -116,6 +134,7 @@ class RasterLayer {
         std::vector<double> values;
         if (!readFileInfo(inputfile, dcmpType, &values)) return false;
         if (!layerDcmp(dcmpType)) return false;
+        retireDevice();
         elemType *dst = _pCellSpace->data();
         for (size_t i = 0; i < values.size(); i++) dst[i] = (elemType)values[i];
         return true;
@@ -130,6 +149,7 @@ class RasterLayer {
         if (!readFileInfo(inputfile, dcmpType, &values)) return false;
         if (!layerDcmp(subWorkBR)) return false;
         const int r0 = _pMetaData->_MBR.minIRow(), cols = _pMetaData->_glbDims.nCols();
+        retireDevice();
         elemType *dst = _pCellSpace->data();
         const size_t n = (size_t)_pMetaData->_localdims.nRows() * cols;
         const double *src = values.data() + (size_t)r0 * cols;
@@ -223,6 +243,7 @@ class RasterLayer {
             cerr << "[ERROR] RasterLayer::writeFile: layer " << _strLayerName << " has no data" << endl;
             return false;
         }
+        if (!syncHost()) return false;
         geotiff::Info info;
         info.rows = _pMetaData->_glbDims.nRows();
         info.cols = _pMetaData->_glbDims.nCols();
@@ -272,6 +293,59 @@ class RasterLayer {
 
     MetaData *_pMetaData = nullptr;  ///< public, as in the reference
 
+    // ---- device residency (double layers) --------------------------------
+    // The current device copy of the cells, or nullptr.
+    const DeviceRaster *deviceCopy() const { return _dev && _dev->current ? _dev.get() : nullptr; }
+    // The cells as float32 on the device, uploaded from the host copy unless
+    // a current float32 device copy exists; nullptr (with a message) on
+    // failure or for a value float32 cannot hold.
+    const DeviceRaster *deviceF32() const
+    {
+        static_assert(std::is_same<elemType, double>::value, "device copies hold double layers");
+        if (_dev && _dev->current && _dev->form == DeviceForm::F32 && !_dev->mask) return _dev.get();
+        if (!syncHost() || !_pCellSpace || !_pMetaData) return nullptr;
+        const SpaceDims &d = _pMetaData->_localdims;
+        if (!_dev) _dev.reset(new DeviceRaster());
+        if (!_dev->allocate(DeviceForm::F32, d.nRows(), d.nCols(), false)) return nullptr;
+        if (!staging::uploadF32(_pCellSpace->data(), (size_t)d.size(), d.nCols(), (float *)_dev->data,
+                                _strLayerName.c_str()))
+            return nullptr;
+        _dev->current = true;
+        _dev->newerThanHost = false;
+        return _dev.get();
+    }
+    // Device buffers of `form` (and a NoData mask) for an operator to write
+    // the whole layer into; call deviceWritten() once it has.
+    DeviceRaster *deviceOutput(DeviceForm form, bool withMask)
+    {
+        static_assert(std::is_same<elemType, double>::value, "device copies hold double layers");
+        if (!_pMetaData) return nullptr;
+        const SpaceDims &d = _pMetaData->_localdims;
+        if (!_dev) _dev.reset(new DeviceRaster());
+        return _dev->allocate(form, d.nRows(), d.nCols(), withMask) ? _dev.get() : nullptr;
+    }
+    // The device copy now holds the cells; the host copy is stale.
+    void deviceWritten()
+    {
+        if (!_dev) return;
+        _dev->current = true;
+        _dev->newerThanHost = true;
+    }
+    // Brings the host CellSpace up to date from a newer device copy.
+    bool syncHost() const
+    {
+        if (!_dev || !_dev->current || !_dev->newerThanHost) return true;
+        if constexpr (std::is_same<elemType, double>::value) {
+            if (!_pCellSpace || _pCellSpace->nRows() != _dev->rows || _pCellSpace->nCols() != _dev->cols)
+                _pCellSpace.reset(new CellSpace<elemType>(SpaceDims(_dev->rows, _dev->cols)));
+            if (!staging::download(*_dev, _pMetaData ? _pMetaData->noData : 0.0, _pCellSpace->data())) return false;
+            _dev->newerThanHost = false;
+            return true;
+        } else {
+            return false;
+        }
+    }
+
   private:
     bool readFileInfo(const char *inputfile, DomDcmpType dcmpType, std::vector<double> *values)
     {
@@ -316,9 +390,19 @@ class RasterLayer {
         os << "<PAMDataset>\n  <SRS>" << wkt << "</SRS>\n</PAMDataset>\n";
     }
 
+    // host data replaced or possibly written: the device copy no longer
+    // holds the cells (its buffers stay for reuse)
+    void retireDevice()
+    {
+        if (_dev) _dev->current = _dev->newerThanHost = false;
+    }
+
     string _strLayerName;
     int _nLayerID = -1;
-    std::unique_ptr<CellSpace<elemType>> _pCellSpace;
+    // mutable: bringing the host copy up to date (syncHost) and making the
+    // device copy (deviceF32) do not change the layer's cells
+    mutable std::unique_ptr<CellSpace<elemType>> _pCellSpace;
+    mutable std::unique_ptr<DeviceRaster> _dev;
     std::unique_ptr<Neighborhood<elemType>> _pNbrhood;
     std::unique_ptr<MetaData> _meta;
 };

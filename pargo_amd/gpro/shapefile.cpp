@@ -8,6 +8,8 @@
 #include <fstream>
 #include <iterator>
 
+#include "../csrc/faults.h"
+
 namespace GPRO {
 namespace shapefile {
 
@@ -17,6 +19,7 @@ bool slurp(const std::string &path, std::vector<unsigned char> &b)
 {
     std::ifstream is(path, std::ios::binary);
     if (!is) return false;
+    pargo::alloc_point("shapefile.file");
     b.assign(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>());
     return true;
 }
@@ -61,6 +64,10 @@ bool read_points(const std::string &path, int field_index, Points &out, std::str
     // DBF header and field descriptors
     const uint32_t nrec = (uint32_t)le32(&dbf[4]);
     const uint16_t hlen = le16(&dbf[8]), rlen = le16(&dbf[10]);
+    if (hlen < 33 || rlen < 1) {
+        err = "bad DBF header in " + base + ".dbf";
+        return false;
+    }
     struct Field {
         int off, len;
         char type;
@@ -83,19 +90,27 @@ bool read_points(const std::string &path, int field_index, Points &out, std::str
         err = "truncated DBF file";
         return false;
     }
+    if (field_index >= 0 && fields[field_index].off + fields[field_index].len > (int)rlen) {
+        err = "DBF field " + std::to_string(field_index) + " lies outside the record";
+        return false;
+    }
     // walk the .shp records alongside the .dbf records
+    pargo::alloc_point("shapefile.points");
+    out.x.reserve(nrec);
+    out.y.reserve(nrec);
+    out.value.reserve(nrec);
     size_t o = 100;
     for (uint32_t rec = 0; rec < nrec; rec++) {
         if (o + 8 > shp.size()) {
             err = "truncated SHP file";
             return false;
         }
-        const int32_t clen = be32(&shp[o + 4]) * 2;
-        const unsigned char *c = &shp[o + 8];
-        if (o + 8 + (size_t)clen > shp.size() || clen < 4) {
+        const int64_t clen = (int64_t)be32(&shp[o + 4]) * 2;  // 16-bit words
+        if (clen < 4 || (uint64_t)clen > shp.size() - (o + 8)) {
             err = "truncated SHP record";
             return false;
         }
+        const unsigned char *c = &shp[o + 8];
         o += 8 + (size_t)clen;
         const unsigned char *r = &dbf[hlen + (size_t)rec * rlen];
         if (r[0] == '*') continue;  // deleted record

@@ -157,6 +157,24 @@ class IDWOperator:
                                         ctypes.byref(c)))
         return out
 
+    ORDER = {"rows": 0, "estimate": 1, "measured": 2}
+
+    def order_tiles(self, source="estimate"):
+        """Launch order of the 16 x 16 cell tiles of every later run, work and
+        adjoint call: "rows" (row-major), "estimate" (heaviest first by the
+        reference's load estimate, IdwTransformation over a ComputeLayer of
+        granularity 16, idwTransformation.cpp:3-62) or "measured" (heaviest
+        first by the samples each tile's searches examined in one pass, the
+        reference's preliminary experiment, idwOperator.cpp:439-491).  The
+        values do not depend on the order."""
+        mr = mc = 0
+        if self.mask is not None:
+            mr, mc = self.mask.shape
+        self.ctx.bind_stream()
+        self.ctx.check(_lib.lib().pargo_idw_order_tiles(self.ctx.h, self.plan, self.k, self.power, self.buffer,
+                                                        _lib._ptr(self.mask), mr, mc, self.mask_nodata,
+                                                        self.ORDER[source]))
+
     def adjoint(self, grad_out):
         """dL/d(sample values) in file order (float64 device tensor)."""
         import torch

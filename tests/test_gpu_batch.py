@@ -130,3 +130,21 @@ def test_mixed_cell_sizes_all_modes_agree():
         ref = single[i][0].double()
         got = outs[0][0][i].double()
         assert torch.all((got - ref).abs() <= 1e-5 * ref.abs() + 1e-12), "tile %d slope vs single call" % i
+
+
+@pytest.mark.parametrize("source", ["rows", "measured", "estimate"])
+def test_queue_order_does_not_change_values(source):
+    """pargo_batch_order (cells, measured per-tile times, or as given) only
+    changes when tiles start: every mode still equals the single calls."""
+    import torch
+    dems = make_tiles(24, 31, 3, 260)
+    ref = eager(dems)
+    b = batch.Batch(dems)
+    b.order(source)
+    for mode in (batch.PER_TILE, batch.PERSISTENT, batch.GRAPH):
+        s, c = b.run(mode)
+        torch.cuda.synchronize()
+        for i in range(len(dems)):
+            assert torch.equal(c[i], ref[i][1]), "%s mode %d tile %d D8" % (source, mode, i)
+            assert torch.equal(s[i], ref[i][0]), "%s mode %d tile %d slope" % (source, mode, i)
+    b.order("estimate")

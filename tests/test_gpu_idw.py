@@ -145,3 +145,23 @@ def test_compute_load_nonterminating_is_rejected():
         op.compute_load(10)
     op.free()
     assert e.value.code == _lib.PARGO_E_NONTERM
+
+
+@pytest.mark.parametrize("source", ["estimate", "measured", "rows"])
+def test_tile_order_keeps_values(source):
+    """pargo_idw_order_tiles: heaviest 16 x 16 tiles first (reference load
+    estimate or a measured pass) changes when tiles start, not what they
+    compute: forward, work and adjoint equal the row-major launch."""
+    import torch
+    xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 3000, width=20000.0, height=15000.0)
+    op = IDWOperator(30.0, 12, 2, 3000.0, 2500.0).read_samples(xs, ys, vs)
+    ref = op.run().clone()
+    ref_work = op.work().clone()
+    g = torch.rand(ref.shape, device="cuda")
+    ref_grad = op.adjoint(g).clone()
+    op.order_tiles(source)
+    assert torch.equal(op.run(), ref)
+    assert torch.equal(op.work(), ref_work)
+    # the adjoint's atomic additions may land in another order
+    assert torch.allclose(op.adjoint(g), ref_grad, rtol=1e-12, atol=1e-14)
+    op.free()
