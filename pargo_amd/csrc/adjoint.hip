@@ -184,7 +184,10 @@ __global__ __launch_bounds__(256) void slope_adjoint_kernel(AdjParams p)
         }
         float *dst = p.gdem + (long long)r * p.pitch + c0;
         if (VEC && full) {
-            *reinterpret_cast<float4 *>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+            // streamed past the caches: the gradient is not read again here
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            const f32x4 t = {out[0], out[1], out[2], out[3]};
+            __builtin_nontemporal_store(t, reinterpret_cast<f32x4 *>(dst));
         } else {
 #pragma unroll
             for (int j = 0; j < ACPT; j++)

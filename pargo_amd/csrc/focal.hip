@@ -100,6 +100,9 @@ struct FocalParams {
     int waves_x;          // batch: wave-wide column bands of this tile
 };
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 template <int CPT>
 struct RawRow {
     float v[CPT];
@@ -550,15 +553,20 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlas
     }
 
     // ---- stores --------------------------------------------------------------
+    // Full-wave rows go out as non-temporal stores: the outputs are not read
+    // again by this pass, and streaming them past the caches measured 1-1.5 %
+    // faster at every strip height on 16384^2 (values unchanged).
     const long long rbase = (long long)r * p.pitch;
     const unsigned uc0 = (unsigned)c0;  // column indexes are never negative
     if (DO_SLOPE) {
         float *srow = p.slope + rbase;
         if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
-                *reinterpret_cast<float4 *>(srow + uc0) = make_float4(sl[0], sl[1], sl[2], sl[3]);
+                const f32x4 t = {sl[0], sl[1], sl[2], sl[3]};
+                __builtin_nontemporal_store(t, reinterpret_cast<f32x4 *>(srow + uc0));
             } else if constexpr (CPT == 2) {
-                *reinterpret_cast<float2 *>(srow + uc0) = make_float2(sl[0], sl[1]);
+                const f32x2 t = {sl[0], sl[1]};
+                __builtin_nontemporal_store(t, reinterpret_cast<f32x2 *>(srow + uc0));
             } else {
 #pragma unroll
                 for (int j = 0; j < CPT; j++) srow[uc0 + j] = sl[j];
@@ -573,11 +581,13 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlas
         uint8_t *crow = p.code + rbase;
         if constexpr (VECLOAD) {
             if constexpr (CPT == 4) {
-                *reinterpret_cast<unsigned *>(crow + uc0) = __builtin_amdgcn_perm(
+                const unsigned packed = __builtin_amdgcn_perm(
                     __builtin_amdgcn_perm(cd[3], cd[2], 0x0c0c0400u), __builtin_amdgcn_perm(cd[1], cd[0], 0x0c0c0400u),
                     0x05040100u);  // byte 0 of each code, three byte permutes
+                __builtin_nontemporal_store(packed, reinterpret_cast<unsigned *>(crow + uc0));
             } else if constexpr (CPT == 2) {
-                *reinterpret_cast<unsigned short *>(crow + uc0) = (unsigned short)(cd[0] | (cd[1] << 8));
+                __builtin_nontemporal_store((unsigned short)(cd[0] | (cd[1] << 8)),
+                                            reinterpret_cast<unsigned short *>(crow + uc0));
             } else {
 #pragma unroll
                 for (int j = 0; j < CPT; j++) crow[uc0 + j] = (uint8_t)cd[j];
