@@ -193,6 +193,53 @@ def secondary_adjoint(torch, _lib, focal, synth):
     return out
 
 
+def secondary_methods(torch, _lib, focal, z, cpu):
+    """Every -mtd slope method on the headline DEM: fused slope+D8
+    (dirType 256) and slope alone, each timed over 10 launches.  With the CPU
+    leg, the oracle's slope of a 256-row band is timed per method (one
+    thread) and the device slope checked against it there (relative 1e-5,
+    NoData set equal)."""
+    dem = torch.from_numpy(z).cuda()
+    rows, cols = z.shape
+    s = torch.empty_like(dem)
+    c = torch.empty(dem.shape, dtype=torch.uint8, device="cuda")
+    ctx = _lib.default_context(0)
+    ctx.bind_stream()
+    L = _lib.lib()
+    grid = focal.grid_of(dem, CELL, NODATA)
+    r0 = rows // 2
+    res = {"config": "slope per -mtd method on the %dx%d headline DEM, tan units; fused = slope + D8 dirType 256"
+                     % (rows, cols)}
+    if cpu is not None:
+        from oracle import pyoracle
+        import numpy as np
+        band = z[r0 - 1:r0 + 257]
+        res["cpu_sample"] = "rows %d..%d (a 258-row band with its halo rows), one thread" % (r0, r0 + 255)
+    names = ("FD", "FFD", "MD", "SD", "SFD", "TFD", "TFDW")
+    for m, name in enumerate(names):
+        fused = event_time(torch, lambda: ctx.check(L.pargo_slope_d8(ctx.h, ctypes.byref(grid), _lib._ptr(dem), m, 0,
+                                                                     256, _lib._ptr(s), _lib._ptr(c), None)), 10)
+        alone = event_time(torch, lambda: ctx.check(L.pargo_slope(ctx.h, ctypes.byref(grid), _lib._ptr(dem), m, 0,
+                                                                  _lib._ptr(s), None)), 10)
+        r = {"fused_ms": round(fused, 4), "fused_Gcells_s": round(rows * cols / fused / 1e6, 1),
+             "slope_ms": round(alone, 4), "slope_Gcells_s": round(rows * cols / alone / 1e6, 1),
+             "slope_GBs": round(8.0 * rows * cols / alone / 1e6, 1)}
+        if cpu is not None:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ref = pyoracle.slope(band, CELL, NODATA, name)[1:-1, 1:-1]
+            dt = time.perf_counter() - t0
+            got = s[r0:r0 + 256, 1:-1].cpu().numpy()
+            nd = ref == NODATA
+            r["cpu_Gcells_s"] = round(256 * (cols - 2) / dt / 1e9, 4)
+            r["band_within_1e-5"] = bool(np.array_equal(got == np.float32(NODATA), nd)) and bool(
+                np.all(np.abs(got[~nd] - ref[~nd]) <= 1e-5 * np.abs(ref[~nd]) + 1e-12))
+        res[name] = r
+    del dem, s, c
+    torch.cuda.empty_cache()
+    return res
+
+
 def secondary_idw(torch, synth, IDWOperator, cpu):
     xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 20000)
     op = IDWOperator(CELL, 12, 2, 5000.0, 5000.0).read_samples(xs, ys, vs)
@@ -433,7 +480,8 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--only", choices=("ceiling", "adjoint", "idw", "batch", "sca", "fill"), default=None,
+    ap.add_argument("--only", choices=("ceiling", "adjoint", "idw", "batch", "sca", "fill", "methods"),
+                    default=None,
                     help="run the headline and this one secondary workload only (for profiling)")
     ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy used to calibrate counters")
     a = ap.parse_args()
@@ -580,8 +628,11 @@ def main():
         if want("batch"):
             sec["batch"] = secondary_batch(torch, batch, cpu)
             log("batch done (%.1fs)" % (time.time() - t0))
-        if want("sca") or want("fill"):
+        if want("sca") or want("fill") or want("methods"):
             zh = synth.workload_dem("slope_d8")
+            if want("methods"):
+                sec["methods"] = secondary_methods(torch, _lib, focal, zh, cpu)
+                log("methods done (%.1fs)" % (time.time() - t0))
             if want("sca"):
                 sec["sca"] = secondary_sca(torch, synth, focal, hydro, zh, cpu)
                 log("sca done (%.1fs)" % (time.time() - t0))

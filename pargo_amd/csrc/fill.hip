@@ -20,12 +20,14 @@
 // reference's while the order is free.
 //
 // Device order: 64 x 32 tiles.  A workgroup loads its tile and a one-cell
-// halo of W into LDS and sweeps the update over the tile (each wave down and
-// up its 8 rows) until nothing in it changes; then it writes the tile back
-// and, for each changed edge, schedules the neighbouring tiles for the next
-// pass.  Passes repeat until one changes nothing.  Tiles read each other's
-// edges while they change; that only feeds the monotone iteration earlier
-// values, never wrong ones.
+// halo of W into LDS (the halo columns from per-tile copies of the
+// neighbours' edge columns) and sweeps the update over the tile (each wave
+// down and up its 8 rows) until nothing in it changes; then it writes the
+// changed cells back (and the edge copies of those in its first or last
+// column) and schedules for the next pass the tile across each changed
+// edge and the diagonal tile at each changed corner.  Passes repeat until
+// one changes nothing.  Tiles read each other's edges while they change;
+// that only feeds the monotone iteration earlier values, never wrong ones.
 //
 // Only scheduled tiles run: a pass walks a device worklist with a fixed
 // grid, and a tile enters the next pass's list when it is first stamped for
@@ -55,6 +57,7 @@ struct FillParams {
     unsigned *__restrict__ changed;  // per pass of the current batch: tiles that changed
     unsigned *__restrict__ count;    // [2]: entries of the worklists of even and odd passes
     int *__restrict__ list;          // [2][tiles]: tiles of the pass (even, odd)
+    double *__restrict__ edge;       // [tiles][2][kTH]: W of each tile's first and last column
     long long pitch;
     int rows, cols, tiles_x, tiles_y;
     float nd_lo, nd_hi;  // float32 values with fabs((double)v - nodata) <= 1e-6
@@ -62,6 +65,14 @@ struct FillParams {
 };
 
 __device__ __forceinline__ bool is_nd(const FillParams &p, float v) { return v >= p.nd_lo && v <= p.nd_hi; }
+
+// Each tile's first and last column of W, also kept contiguously per tile:
+// a tile's halo columns are its neighbours' edge columns, read as two short
+// runs instead of one extra cache line per halo row on each side.
+__device__ __forceinline__ long long edge_index(const FillParams &p, int ty, int tx, int right, int row_in_tile)
+{
+    return (((long long)ty * p.tiles_x + tx) * 2 + right) * kTH + row_in_tile;
+}
 
 // Puts tile t on the worklist of pass `pass` unless it is there already
 // (its stamp says so).
@@ -83,7 +94,10 @@ __global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
     const long long i = (long long)r * p.pitch + c;
     const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
     const bool valid = interior && !is_nd(p, p.dem[i]);
-    p.w[i] = valid ? kStart : p.nodata;
+    const double w0 = valid ? kStart : p.nodata;
+    p.w[i] = w0;
+    if (c % kTW == 0 || c % kTW == kTW - 1)
+        p.edge[edge_index(p, r / kTH, c / kTW, c % kTW != 0, r % kTH)] = w0;
     // The tiles whose cells or halo hold an excluded cell of this wave.  A
     // wave is 64 columns of one row starting at a multiple of kTW, so they
     // are its own tile column w, w - 1 when its first cell is excluded and
@@ -132,47 +146,87 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
     if (!run) return;  // moved on to the next pass meanwhile
     const int ty = tile / p.tiles_x, tx = tile - ty * p.tiles_x;
     const int r0 = ty * kTH, c0 = tx * kTW;
-    for (int i = tid; i < (kTH + 2) * (kTW + 2); i += 256) {
-        const int lr = i / (kTW + 2), lc = i % (kTW + 2);
-        const int r = r0 - 1 + lr, c = c0 - 1 + lc;
-        sw[lr][lc] = (r >= 0 && r < p.rows && c >= 0 && c < p.cols) ? p.w[(long long)r * p.pitch + c] : p.nodata;
+    {
+        // Rows r0-1 .. r0+kTH of the tile's columns from W (aligned runs of
+        // kTW values), the halo columns from the neighbours' edge columns.
+        // Every load of the thread is issued before the first store to LDS,
+        // so that their latencies overlap.
+        constexpr int N = (kTH + 2) * kTW, PER = (N + 255) / 256;
+        double v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * 256;
+            const int r = r0 - 1 + i / kTW, c = c0 + i % kTW;
+            v[k] = (i < N && r >= 0 && r < p.rows && c < p.cols) ? p.w[(long long)r * p.pitch + c] : p.nodata;
+        }
+        // threads 0 .. 2(kTH+2)-1: left halo column, then right
+        double h = p.nodata;
+        const int side = tid / (kTH + 2), hr = tid % (kTH + 2);
+        if (side < 2) {
+            const int r = r0 - 1 + hr, c = side == 0 ? c0 - 1 : c0 + kTW;
+            if (r >= 0 && r < p.rows && c >= 0 && c < p.cols)
+                h = p.edge[edge_index(p, r / kTH, c / kTW, side == 0, r % kTH)];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * 256;
+            if (i < N) sw[i / kTW][i % kTW + 1] = v[k];
+        }
+        if (side < 2) sw[hr][side == 0 ? 0 : kTW + 1] = h;
     }
     // own cells: wave w owns tile rows 8w .. 8w+7, lane = column
     const int lc = tid % kTW + 1;
     const int rbase = (tid / kTW) * kRW;  // first tile row of this wave
-    double d[kRW];
+    float dem_f[kRW];  // the cells' DEM values (widened to double where used)
     unsigned live = 0;  // bit k: tile row rbase + k holds a valid interior cell
 #pragma unroll
     for (int k = 0; k < kRW; k++) {
         const int r = r0 + rbase + k, c = c0 + lc - 1;
-        d[k] = 0.0;
+        dem_f[k] = 0.0f;
         if (r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2) {
             const float f = p.dem[(long long)r * p.pitch + c];
             if (!is_nd(p, f)) {
-                d[k] = (double)f;
+                dem_f[k] = f;
                 live |= 1u << k;
             }
         }
     }
     __syncthreads();
+    // A NoData neighbour sends a cell to its DEM value whatever the other
+    // neighbours hold.  NoData cells keep W = NoData and no valid W comes
+    // within 1e-6 of it, so which cells have one is fixed during the visit.
+    unsigned nd_nbr = 0;  // bit k: the cell in tile row rbase + k has a NoData neighbour
+#pragma unroll
+    for (int k = 0; k < kRW; k++) {
+        const int lr = rbase + k + 1;
+        bool any = false;
+#pragma unroll
+        for (int n = 0; n < 9; n++)
+            if (n != 4) any |= fabs(sw[lr + n / 3 - 1][lc + n % 3 - 1] - p.nodata) < 0.000001;
+        nd_nbr |= (unsigned)any << k;
+    }
     unsigned touched = 0;  // bit k: the cell changed during this pass
     // One update of the cell in tile row rbase + k; true if it changed.
+    // Without a NoData neighbour the reference's window loop gives
+    // W = dem if dem >= W(n) + gap for some n, else min(W, min_n W(n) + gap);
+    // the rounded addition is monotone, so min_n (W(n) + gap) is
+    // (min_n W(n)) + gap with one rounding, and "for some n" is a test
+    // against that minimum.
     auto update = [&](int k) -> bool {
         if (!((live >> k) & 1u)) return false;
         const int lr = rbase + k + 1;
         const double w = sw[lr][lc];
-        if (w <= d[k]) return false;
-        bool to_dem = false;
-        double m = w;
+        const double dk = (double)dem_f[k];
+        if (w <= dk) return false;
+        double nw = dk;
+        if (!((nd_nbr >> k) & 1u)) {
+            double wmin = sw[lr - 1][lc - 1];
 #pragma unroll
-        for (int n = 0; n < 9; n++) {
-            if (n == 4) continue;
-            const double wn = sw[lr + n / 3 - 1][lc + n % 3 - 1];
-            const double s = wn + kGap;
-            if (d[k] >= s || fabs(wn - p.nodata) < 0.000001) to_dem = true;
-            else if (m > s) m = s;
+            for (int n = 1; n < 9; n++)
+                if (n != 4) wmin = fmin(wmin, sw[lr + n / 3 - 1][lc + n % 3 - 1]);
+            const double sm = wmin + kGap;
+            nw = dk >= sm ? dk : fmin(w, sm);
         }
-        const double nw = to_dem ? d[k] : m;
         if (nw == w) return false;
         sw[lr][lc] = nw;
         touched |= 1u << k;
@@ -203,8 +257,14 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
         if (!((touched >> k) & 1u)) continue;
         const int tr = rbase + k, tc = lc - 1;
         const int r = r0 + tr, c = c0 + tc;
-        p.w[(long long)r * p.pitch + c] = sw[tr + 1][lc];
-        flags |= (tr == 0 ? 1u : 0u) | (tr == kTH - 1 ? 2u : 0u) | (tc == 0 ? 4u : 0u) | (tc == kTW - 1 ? 8u : 0u) | 16u;
+        const double nw = sw[tr + 1][lc];
+        p.w[(long long)r * p.pitch + c] = nw;
+        if (tc == 0 || tc == kTW - 1) p.edge[edge_index(p, ty, tx, tc != 0, tr)] = nw;
+        const unsigned top = tr == 0, bottom = tr == kTH - 1, left = tc == 0, right = tc == kTW - 1;
+        // edges (bits 0-3) schedule the tile across them; a corner (bits
+        // 4-7) is the only cell in a diagonal neighbour's halo
+        flags |= top | (bottom << 1) | (left << 2) | (right << 3) | ((top & left) << 4) | ((top & right) << 5) |
+                 ((bottom & left) << 6) | ((bottom & right) << 7) | (1u << 8);
     }
     if (flags) atomicOr(&edge_flags, (int)flags);
     __syncthreads();
@@ -215,8 +275,10 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
         for (int dy = -1; dy <= 1; dy++) {
             for (int dx = -1; dx <= 1; dx++) {
                 if (dx == 0 && dy == 0) continue;
-                const bool need = (dy == -1 && (f & 1)) || (dy == 1 && (f & 2)) || (dx == -1 && (f & 4)) ||
-                                  (dx == 1 && (f & 8));
+                const int corner = 4 + (dy == 1 ? 2 : 0) + (dx == 1 ? 1 : 0);
+                const bool need = dx == 0   ? (f >> (dy == -1 ? 0 : 1)) & 1
+                                  : dy == 0 ? (f >> (dx == -1 ? 2 : 3)) & 1
+                                            : (f >> corner) & 1;
                 const int nty = ty + dy, ntx = tx + dx;
                 if (need && nty >= 0 && nty < p.tiles_y && ntx >= 0 && ntx < p.tiles_x)
                     schedule(p, nty * p.tiles_x + ntx, next);
@@ -226,7 +288,7 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
 }
 
 
-__global__ __launch_bounds__(256) void fill_pass_kernel(FillParams p, int pass, int batch_first)
+__global__ __launch_bounds__(256, 5) void fill_pass_kernel(FillParams p, int pass, int batch_first)
 {
     __shared__ double sw[kTH + 2][kTW + 2];
     __shared__ int edge_flags, run;
@@ -320,7 +382,8 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
     // scratch: [0] min bits, [1] NaN count, per-pass counters, the two
     // worklist counts, then stamps and the two worklists
     const size_t counters = 2 + (size_t)kBatch + 2;
-    rc = pargo::ensure_scratch(ctx, sizeof(unsigned) * counters + sizeof(int) * 3 * (size_t)ntiles + 256);
+    const size_t edge_off = (sizeof(unsigned) * counters + sizeof(int) * 3 * (size_t)ntiles + 255) / 256 * 256;
+    rc = pargo::ensure_scratch(ctx, edge_off + sizeof(double) * 2 * pargo::kTH * (size_t)ntiles);
     if (rc) return rc;
     unsigned *scr = static_cast<unsigned *>(ctx->scratch);
     unsigned *minbits = scr, *nans = scr + 1, *changed = scr + 2;
@@ -328,6 +391,7 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
     p.count = scr + 2 + kBatch;
     p.stamp = reinterpret_cast<int *>(scr + counters);
     p.list = p.stamp + ntiles;
+    p.edge = reinterpret_cast<double *>(static_cast<char *>(ctx->scratch) + edge_off);
 
     // monotonicity check: every valid W lies in [min(dem, 10000), 10000]
     unsigned host2[2] = {0xffffffffu, 0u};

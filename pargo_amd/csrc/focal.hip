@@ -84,6 +84,7 @@ struct FocalParams {
     float ndu_lo, ndu_hi; // smallest interval holding both (+inf,-inf if empty)
     float slope_nodata;   // value written to NoData slope cells
     double inv64cs2;      // 1 / (64 cs^2) for the Horn fast path
+    double fast_scale;    // other methods' fast path: 1 / (k cs)^2 (MD: 1 / (cs sqrt 2)); 0: per cell in FP64
     double cs;            // float-rounded cell size
     int method;
     int unit;
@@ -386,6 +387,56 @@ __device__ __forceinline__ unsigned d8_window_fp64(double d0, double d1, double 
     return (M < kD8Eps) ? 0u : c;
 }
 
+// The same methods with the two divisions and the FP64 root replaced: the
+// numerators of dx and dy are the reference's FP64 expressions, in its
+// order, over the float-widened window; the sum of their squares is scaled
+// once by 1 / (k cs)^2 (k = 8, 4, 1, 2, 6 or 4 + sqrt 8 as the method's
+// divisor has it) and its root taken in FP32.  MD's maximum drop is scaled
+// by 1 / (cs sqrt 2).  Against the reference's double result this moves the
+// float output by at most about one float ulp.  With |z| < 2^40 (larger
+// windows go per cell) and 2^-20 <= cs <= 2^40 the scaled sum stays below
+// 2^123, inside the float range.
+__device__ __forceinline__ float slope_fast(const double d[9], int method, double scale)
+{
+    double nx, ny;
+    switch (method) {
+    case PARGO_SLOPE_FFD:
+        nx = d[8] - d[6] + d[2] - d[0];
+        ny = d[0] - d[6] + d[2] - d[8];
+        break;
+    case PARGO_SLOPE_MD: {
+        double m = 0.;
+        const int order[8] = {0, 1, 2, 3, 5, 6, 7, 8};
+        for (int t = 0; t < 8; t++) {
+            const double v = d[4] - d[order[t]];
+            if (m < v) m = v;
+        }
+        return (float)(m * scale);
+    }
+    case PARGO_SLOPE_SD:
+        nx = d[4] - d[3];
+        ny = d[4] - d[7];
+        break;
+    case PARGO_SLOPE_SFD:
+        nx = d[5] - d[3];
+        ny = d[1] - d[7];
+        break;
+    case PARGO_SLOPE_TFD:
+        nx = d[8] + d[5] + d[2] - d[0] - d[3] - d[6];
+        ny = d[2] + d[1] + d[0] - d[6] - d[7] - d[6];  // d[6] twice, as the reference
+        break;
+    case PARGO_SLOPE_TFDW:
+        nx = d[8] + kSqrt2 * d[5] + d[2] - d[6] - kSqrt2 * d[3] - d[0];
+        ny = d[2] + kSqrt2 * d[1] + d[0] - d[6] - kSqrt2 * d[7] - d[8];
+        break;
+    default:  // PARGO_SLOPE_FD
+        nx = d[8] + 2 * d[5] + d[2] - d[6] - 2 * d[3] - d[0];
+        ny = d[2] + 2 * d[1] + d[0] - d[6] - 2 * d[7] - d[8];
+        break;
+    }
+    return __builtin_amdgcn_sqrtf((float)((nx * nx + ny * ny) * scale));
+}
+
 __device__ __forceinline__ float slope_unit(float s, int unit)
 {
     return unit == PARGO_UNIT_DEGREES ? atanf(s) * 57.295779513082323f : s;
@@ -398,19 +449,19 @@ __device__ __forceinline__ float slope_unit(float s, int unit)
 // `special` is set when a window of this lane holds a value the fast path
 // does not handle.  MASK false compiles the NoData mask output out (p.mask
 // is then ignored); MASK true writes it when p.mask is set.
-template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK>
+template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, int SM, bool MASK>
 __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlast, int c0, int lane, unsigned colb,
                                            Slot<CPT> &up, Slot<CPT> &mid, Slot<CPT> &dn, RawRow<CPT> &raw,
                                            bool &special)
 {
-    finish_row<CPT, DO_SLOPE && !HORN>(p, raw, lane, dn);
+    finish_row<CPT, DO_SLOPE && SM == 0>(p, raw, lane, dn);
     load_raw<CPT, VECLOAD>(p, min(r + 4, rlast), c0, lane, raw);
 
     float sl[CPT];
     unsigned cd[CPT];
     // ---- slope ------------------------------------------------------------
     if (DO_SLOPE) {
-        if (HORN) {
+        if constexpr (SM == 1) {
             // Horn numerators: nx from column sums up + 2 mid + dn, ny from the
             // vertical differences up - dn (both FP64; the float inputs make
             // them exact or within an ulp of the reference's sums)
@@ -427,6 +478,15 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlas
                 const double ny = __builtin_fma(2.0, vd[j + 1], vd[j] + vd[j + 2]);
                 const double q = __builtin_fma(nx, nx, ny * ny) * p.inv64cs2;
                 sl[j] = __builtin_amdgcn_sqrtf((float)q);
+            }
+        } else if constexpr (SM == 2) {
+            // the other methods' fast path on the float-widened window
+#pragma unroll
+            for (int j = 0; j < CPT; j++) {
+                const int i = j + 1;
+                const double d[9] = {up.zf[i - 1], up.zf[i], up.zf[i + 1], mid.zf[i - 1], mid.zf[i],
+                                     mid.zf[i + 1], dn.zf[i - 1], dn.zf[i], dn.zf[i + 1]};
+                sl[j] = slope_fast(d, p.method, p.fast_scale);
             }
         } else {
 #pragma unroll
@@ -618,7 +678,7 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlas
 }
 
 // Pairs from a row to the row below (the prologue of a strip).
-template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool DO_SLOPE, bool DO_D8, int SM>
 __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
 {
     if (DO_D8) {
@@ -633,7 +693,7 @@ __device__ __forceinline__ void pair_rows(Slot<CPT> &up, const Slot<CPT> &below)
 
 // One wave computes output rows [rbeg, rend) of the 64*CPT columns that
 // start at wave_c0 (wave-uniform).
-template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK>
+template <int CPT, bool VECLOAD, bool DO_SLOPE, bool DO_D8, int SM, bool MASK>
 __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
 {
     const int c0 = wave_c0 + lane * CPT;
@@ -650,18 +710,18 @@ __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_
         load_raw<CPT, VECLOAD>(p, min(rbeg + 1, rend), c0, lane, R0);
         load_raw<CPT, VECLOAD>(p, min(rbeg + 2, rend), c0, lane, R1);
         load_raw<CPT, VECLOAD>(p, min(rbeg + 3, rend), c0, lane, R2);
-        finish_row<CPT, DO_SLOPE && !HORN>(p, ra, lane, A);
-        finish_row<CPT, DO_SLOPE && !HORN>(p, rb, lane, B);
+        finish_row<CPT, DO_SLOPE && SM == 0>(p, ra, lane, A);
+        finish_row<CPT, DO_SLOPE && SM == 0>(p, rb, lane, B);
     }
-    pair_rows<CPT, DO_SLOPE, DO_D8, HORN>(A, B);
+    pair_rows<CPT, DO_SLOPE, DO_D8, SM>(A, B);
     bool special = false;
     // Three slots rotate roles statically: (up, mid, dn) = (A,B,C), (B,C,A), (C,A,B).
     for (int r = rbeg; r < rend; r += 3) {
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r, rend, c0, lane, colb, A, B, C, R0, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, SM, MASK>(p, r, rend, c0, lane, colb, A, B, C, R0, special);
         if (r + 1 >= rend) break;
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 1, rend, c0, lane, colb, B, C, A, R1, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, SM, MASK>(p, r + 1, rend, c0, lane, colb, B, C, A, R1, special);
         if (r + 2 >= rend) break;
-        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, HORN, MASK>(p, r + 2, rend, c0, lane, colb, C, A, B, R2, special);
+        focal_step<CPT, VECLOAD, DO_SLOPE, DO_D8, SM, MASK>(p, r + 2, rend, c0, lane, colb, C, A, B, R2, special);
     }
     // queue the strip for the exact pass (rare: Inf or |z| >= 2^40 nearby)
     if (__builtin_amdgcn_ballot_w64(special) != 0 && lane == 0) {
@@ -672,20 +732,20 @@ __device__ __forceinline__ void focal_strip_impl(const FocalParams &p, int wave_
 
 // VEC: the raster's pitch and base allow vector access; a wave uses it when
 // all its columns are inside the raster.
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK = true>
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, int SM, bool MASK = true>
 __device__ __forceinline__ void focal_strip(const FocalParams &p, int wave_c0, int lane, int rbeg, int rend)
 {
     if constexpr (VEC) {
         if (wave_c0 + 64 * CPT <= p.cols) {
-            focal_strip_impl<CPT, true, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
+            focal_strip_impl<CPT, true, DO_SLOPE, DO_D8, SM, MASK>(p, wave_c0, lane, rbeg, rend);
             return;
         }
     }
-    focal_strip_impl<CPT, false, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
+    focal_strip_impl<CPT, false, DO_SLOPE, DO_D8, SM, MASK>(p, wave_c0, lane, rbeg, rend);
 }
 
 // One raster: block = 4 waves side by side, one row strip per block.
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN, bool MASK = true>
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, int SM, bool MASK = true>
 __device__ __forceinline__ void focal_raster(const FocalParams &p)
 {
     const int lane = threadIdx.x & 63;
@@ -701,13 +761,13 @@ __device__ __forceinline__ void focal_raster(const FocalParams &p)
     const int wave_c0 = (int)(cb * 4 + wave) * 64 * CPT;
     const int rbeg = (int)strip * p.rows_per_strip;
     const int rend = min(rbeg + p.rows_per_strip, p.rows);
-    focal_strip<CPT, VEC, DO_SLOPE, DO_D8, HORN, MASK>(p, wave_c0, lane, rbeg, rend);
+    focal_strip<CPT, VEC, DO_SLOPE, DO_D8, SM, MASK>(p, wave_c0, lane, rbeg, rend);
 }
 
-template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool VEC, bool DO_SLOPE, bool DO_D8, int SM>
 __global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
 {
-    focal_raster<CPT, VEC, DO_SLOPE, DO_D8, HORN>(p);
+    focal_raster<CPT, VEC, DO_SLOPE, DO_D8, SM>(p);
 }
 
 // The fused Horn slope + D8 pass without a NoData mask, at four waves per
@@ -721,13 +781,13 @@ __global__ __launch_bounds__(256) void focal_kernel(FocalParams p)
 template <int CPT, bool VEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void focal_kernel_fused(FocalParams p)
 {
-    focal_raster<CPT, VEC, true, true, true, false>(p);
+    focal_raster<CPT, VEC, true, true, 1, false>(p);
 }
 
 template <int CPT, bool VEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void focal_kernel_fused_mask(FocalParams p)
 {
-    focal_raster<CPT, VEC, true, true, true, true>(p);
+    focal_raster<CPT, VEC, true, true, 1, true>(p);
 }
 
 // Many rasters in one persistent launch: every wave takes (tile, column
@@ -735,7 +795,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // numbered tile by tile (first[t] = first item of tile t; tiles are in
 // decreasing size so that the tail of the queue holds the small items).
 // Batch tiles carry no NoData mask, so the mask code is compiled out.
-template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool DO_SLOPE, bool DO_D8, int SM>
 __device__ __forceinline__ void focal_batch_body(const FocalParams *__restrict__ tiles,
                                                  const unsigned *__restrict__ first, int ntiles,
                                                  unsigned *__restrict__ queue, unsigned total)
@@ -765,17 +825,17 @@ __device__ __forceinline__ void focal_batch_body(const FocalParams *__restrict__
         const int wx = (int)(local % (unsigned)p.waves_x), strip = (int)(local / (unsigned)p.waves_x);
         const int rbeg = strip * p.rows_per_strip;
         const int rend = min(rbeg + p.rows_per_strip, p.rows);
-        if (p.vec) focal_strip<CPT, true, DO_SLOPE, DO_D8, HORN, false>(p, wx * 64 * CPT, lane, rbeg, rend);
-        else focal_strip<CPT, false, DO_SLOPE, DO_D8, HORN, false>(p, wx * 64 * CPT, lane, rbeg, rend);
+        if (p.vec) focal_strip<CPT, true, DO_SLOPE, DO_D8, SM, false>(p, wx * 64 * CPT, lane, rbeg, rend);
+        else focal_strip<CPT, false, DO_SLOPE, DO_D8, SM, false>(p, wx * 64 * CPT, lane, rbeg, rend);
     }
 }
 
-template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool DO_SLOPE, bool DO_D8, int SM>
 __global__ __launch_bounds__(256) void focal_batch_kernel(const FocalParams *__restrict__ tiles,
                                                           const unsigned *__restrict__ first, int ntiles,
                                                           unsigned *__restrict__ queue, unsigned total)
 {
-    focal_batch_body<CPT, DO_SLOPE, DO_D8, HORN>(tiles, first, ntiles, queue, total);
+    focal_batch_body<CPT, DO_SLOPE, DO_D8, SM>(tiles, first, ntiles, queue, total);
 }
 
 // The fused Horn slope + D8 batch at 4 columns per lane runs at four waves
@@ -862,7 +922,7 @@ struct LaunchCfg {
     int rows_per_strip;
 };
 
-template <bool DO_SLOPE, bool DO_D8, bool HORN>
+template <bool DO_SLOPE, bool DO_D8, int SM>
 static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipStream_t s)
 {
     const int threads = 256;
@@ -870,7 +930,7 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
     const int cols_per_block = threads * cfg.cpt;
     dim3 grid((p.cols + cols_per_block - 1) / cols_per_block, (p.rows + cfg.rows_per_strip - 1) / cfg.rows_per_strip);
     if (cfg.cpt == 4) {
-        if constexpr (DO_SLOPE && DO_D8 && HORN) {
+        if constexpr (DO_SLOPE && DO_D8 && SM == 1) {
             if (p.mask) {
                 if (vec) hipLaunchKernelGGL((focal_kernel_fused_mask<4, true>), grid, block, 0, s, p);
                 else hipLaunchKernelGGL((focal_kernel_fused_mask<4, false>), grid, block, 0, s, p);
@@ -879,12 +939,12 @@ static void launch_t(const FocalParams &p, const LaunchCfg &cfg, bool vec, hipSt
                 else hipLaunchKernelGGL((focal_kernel_fused<4, false>), grid, block, 0, s, p);
             }
         } else {
-            if (vec) hipLaunchKernelGGL((focal_kernel<4, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
-            else hipLaunchKernelGGL((focal_kernel<4, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
+            if (vec) hipLaunchKernelGGL((focal_kernel<4, true, DO_SLOPE, DO_D8, SM>), grid, block, 0, s, p);
+            else hipLaunchKernelGGL((focal_kernel<4, false, DO_SLOPE, DO_D8, SM>), grid, block, 0, s, p);
         }
     } else {
-        if (vec) hipLaunchKernelGGL((focal_kernel<2, true, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((focal_kernel<2, false, DO_SLOPE, DO_D8, HORN>), grid, block, 0, s, p);
+        if (vec) hipLaunchKernelGGL((focal_kernel<2, true, DO_SLOPE, DO_D8, SM>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((focal_kernel<2, false, DO_SLOPE, DO_D8, SM>), grid, block, 0, s, p);
     }
 }
 
@@ -898,32 +958,37 @@ static void launch_fixup(const FocalParams &p, const FocalParams *tiles, int nti
     else hipLaunchKernelGGL((focal_fixup_kernel<false, true>), grid, block, 0, s, p, tiles, ntiles, width);
 }
 
-// The Horn fast path forms (nx^2 + ny^2) / (64 cs^2) in FP64 and takes the
-// root in FP32; with |z| < 2^40 (larger values go per cell) and
-// 2^-20 <= cs <= 2^40 that quotient stays below 2^127.
-static bool horn_ok(const FocalParams &p)
+// Slope mode of the kernels: 1 the Horn fast path (FD), 2 the other
+// methods' fast path, 0 every cell in FP64 with the reference's divisions.
+// The fast paths form the scaled sum of squares in FP64 and take the root in
+// FP32; with |z| < 2^40 (larger values go per cell) and 2^-20 <= cs <= 2^40
+// it stays below 2^127 (fast_scale is 0 outside that range).
+static int slope_mode(const FocalParams &p)
 {
-    return p.method == PARGO_SLOPE_FD && p.cs >= 0x1p-20 && p.cs <= 0x1p40;
+    if (!(p.fast_scale > 0.0)) return 0;
+    return p.method == PARGO_SLOPE_FD ? 1 : 2;
 }
 
 static void launch_focal_h(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
-                           bool horn, hipStream_t s)
+                           int sm, hipStream_t s)
 {
     if (do_slope && do_d8) {
-        if (horn) launch_t<true, true, true>(p, cfg, vec, s);
-        else launch_t<true, true, false>(p, cfg, vec, s);
+        if (sm == 1) launch_t<true, true, 1>(p, cfg, vec, s);
+        else if (sm == 2) launch_t<true, true, 2>(p, cfg, vec, s);
+        else launch_t<true, true, 0>(p, cfg, vec, s);
     } else if (do_slope) {
-        if (horn) launch_t<true, false, true>(p, cfg, vec, s);
-        else launch_t<true, false, false>(p, cfg, vec, s);
+        if (sm == 1) launch_t<true, false, 1>(p, cfg, vec, s);
+        else if (sm == 2) launch_t<true, false, 2>(p, cfg, vec, s);
+        else launch_t<true, false, 0>(p, cfg, vec, s);
     } else {
-        launch_t<false, true, true>(p, cfg, vec, s);
+        launch_t<false, true, 1>(p, cfg, vec, s);
     }
 }
 
 static void launch_focal(const FocalParams &p, const LaunchCfg &cfg, bool vec, bool do_slope, bool do_d8,
                          hipStream_t s)
 {
-    launch_focal_h(p, cfg, vec, do_slope, do_d8, horn_ok(p), s);
+    launch_focal_h(p, cfg, vec, do_slope, do_d8, slope_mode(p), s);
 }
 
 static LaunchCfg default_cfg(int rows)
@@ -1041,6 +1106,21 @@ static int prepare(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int me
     p.slope_nodata = (float)g->nodata;
     p.cs = (double)csf;
     p.inv64cs2 = 1.0 / (64.0 * (double)csf * (double)csf);
+    p.fast_scale = 0.0;
+    if (p.cs >= 0x1p-20 && p.cs <= 0x1p40) {
+        if (method == PARGO_SLOPE_MD) {
+            p.fast_scale = 1.0 / (p.cs * kSqrt2);
+        } else {
+            const double k = method == PARGO_SLOPE_FFD    ? 4.0
+                             : method == PARGO_SLOPE_SD   ? 1.0
+                             : method == PARGO_SLOPE_SFD  ? 2.0
+                             : method == PARGO_SLOPE_TFD  ? 6.0
+                             : method == PARGO_SLOPE_TFDW ? 4.0 + 0x1.6a09e667f3bcdp+1
+                                                          : 8.0;
+            const double kcs = k * p.cs;
+            p.fast_scale = 1.0 / (kcs * kcs);
+        }
+    }
     p.method = method;
     p.unit = unit;
     p.dirtype = code_out ? dirtype : 0;
@@ -1098,7 +1178,8 @@ struct BatchPlan {
     int ntiles = 0;
     int cpt = 4;
     int blocks = 0;
-    bool do_slope = false, do_d8 = false, horn = false;
+    bool do_slope = false, do_d8 = false;
+    int sm = 0;  // slope mode shared by every tile (slope_mode)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 };
@@ -1115,28 +1196,28 @@ void free_batch_plan(pargo_ctx *ctx)
     ctx->batch = nullptr;
 }
 
-template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool DO_SLOPE, bool DO_D8, int SM>
 static int batch_blocks(pargo_ctx *ctx)
 {
     int per_cu = 0;
     hipError_t e;
-    if constexpr (CPT == 4 && DO_SLOPE && DO_D8 && HORN)
+    if constexpr (CPT == 4 && DO_SLOPE && DO_D8 && SM == 1)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel_fused, 256, 0);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>, 256, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, focal_batch_kernel<CPT, DO_SLOPE, DO_D8, SM>, 256, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 2;
     const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
     return per_cu * cus;
 }
 
-template <int CPT, bool DO_SLOPE, bool DO_D8, bool HORN>
+template <int CPT, bool DO_SLOPE, bool DO_D8, int SM>
 static void launch_batch_t(const BatchPlan &b, hipStream_t s)
 {
-    if constexpr (CPT == 4 && DO_SLOPE && DO_D8 && HORN)
+    if constexpr (CPT == 4 && DO_SLOPE && DO_D8 && SM == 1)
         hipLaunchKernelGGL(focal_batch_kernel_fused, dim3(b.blocks), dim3(256), 0, s, (const FocalParams *)b.d_tiles,
                            (const unsigned *)b.d_first, b.ntiles, b.d_queue, b.total);
     else
-        hipLaunchKernelGGL((focal_batch_kernel<CPT, DO_SLOPE, DO_D8, HORN>), dim3(b.blocks), dim3(256), 0, s,
+        hipLaunchKernelGGL((focal_batch_kernel<CPT, DO_SLOPE, DO_D8, SM>), dim3(b.blocks), dim3(256), 0, s,
                            (const FocalParams *)b.d_tiles, (const unsigned *)b.d_first, b.ntiles, b.d_queue, b.total);
 }
 
@@ -1144,22 +1225,30 @@ template <int CPT>
 static void launch_batch_c(const BatchPlan &b, hipStream_t s)
 {
     if (b.do_slope && b.do_d8) {
-        if (b.horn) launch_batch_t<CPT, true, true, true>(b, s);
-        else launch_batch_t<CPT, true, true, false>(b, s);
+        if (b.sm == 1) launch_batch_t<CPT, true, true, 1>(b, s);
+        else if (b.sm == 2) launch_batch_t<CPT, true, true, 2>(b, s);
+        else launch_batch_t<CPT, true, true, 0>(b, s);
     } else if (b.do_slope) {
-        if (b.horn) launch_batch_t<CPT, true, false, true>(b, s);
-        else launch_batch_t<CPT, true, false, false>(b, s);
+        if (b.sm == 1) launch_batch_t<CPT, true, false, 1>(b, s);
+        else if (b.sm == 2) launch_batch_t<CPT, true, false, 2>(b, s);
+        else launch_batch_t<CPT, true, false, 0>(b, s);
     } else {
-        launch_batch_t<CPT, false, true, true>(b, s);
+        launch_batch_t<CPT, false, true, 1>(b, s);
     }
 }
 
 template <int CPT>
 static int blocks_c(pargo_ctx *ctx, const BatchPlan &b)
 {
-    if (b.do_slope && b.do_d8) return b.horn ? batch_blocks<CPT, true, true, true>(ctx) : batch_blocks<CPT, true, true, false>(ctx);
-    if (b.do_slope) return b.horn ? batch_blocks<CPT, true, false, true>(ctx) : batch_blocks<CPT, true, false, false>(ctx);
-    return batch_blocks<CPT, false, true, true>(ctx);
+    if (b.do_slope && b.do_d8)
+        return b.sm == 1   ? batch_blocks<CPT, true, true, 1>(ctx)
+               : b.sm == 2 ? batch_blocks<CPT, true, true, 2>(ctx)
+                           : batch_blocks<CPT, true, true, 0>(ctx);
+    if (b.do_slope)
+        return b.sm == 1   ? batch_blocks<CPT, true, false, 1>(ctx)
+               : b.sm == 2 ? batch_blocks<CPT, true, false, 2>(ctx)
+                           : batch_blocks<CPT, true, false, 0>(ctx);
+    return batch_blocks<CPT, false, true, 1>(ctx);
 }
 
 // The persistent sequence: queue reset, batch kernel, fixup.
@@ -1217,12 +1306,12 @@ static int build_batch(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
     }
     b->do_slope = do_slope;
     b->do_d8 = do_d8;
-    // One slope path for the whole batch, in every mode: Horn's fast path
-    // only when every tile's cell size allows it (horn_ok), so that the
+    // One slope path for the whole batch, in every mode: a fast path only
+    // when every tile's cell size allows it (slope_mode), so that the
     // per-tile, persistent and graph modes give the same bits.
-    b->horn = true;
-    for (int t = 0; t < ntiles; t++)
-        if (!horn_ok(b->h_tiles[t])) b->horn = false;
+    b->sm = slope_mode(b->h_tiles[0]);
+    for (int t = 1; t < ntiles; t++)
+        if (slope_mode(b->h_tiles[t]) != b->sm) b->sm = 0;
     // device order, heaviest first by the load model of ctx->batch_order:
     // cells (the intensity-ratio load with equal NoData and valid weights,
     // which is this kernel's cost: a NoData cell costs what a valid one
@@ -1284,7 +1373,7 @@ static int run_batch(pargo_ctx *ctx, BatchPlan *b, int mode, int rows_per_strip)
         LaunchCfg cfg{b->cpt, rows_per_strip};
         for (int t = 0; t < b->ntiles; t++) {
             const FocalParams &p = b->h_tiles[t];
-            launch_focal_h(p, cfg, p.vec != 0, b->do_slope, b->do_d8, b->horn, s);
+            launch_focal_h(p, cfg, p.vec != 0, b->do_slope, b->do_d8, b->sm, s);
             launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, b->cpt, s);
         }
         return check_launch(ctx, "pargo_batch_slope_d8 (per tile)");
@@ -1433,7 +1522,7 @@ static int batch_order(pargo_ctx *ctx, const pargo_tile *tiles, int ntiles, int 
         for (int t = 0; t < ntiles && e == hipSuccess; t++) {
             const pargo::FocalParams &p = b->h_tiles[t];
             e = hipEventRecord(ev[2 * t], ctx->stream);
-            pargo::launch_focal_h(p, cfg, p.vec != 0, b->do_slope, b->do_d8, b->horn, ctx->stream);
+            pargo::launch_focal_h(p, cfg, p.vec != 0, b->do_slope, b->do_d8, b->sm, ctx->stream);
             pargo::launch_fixup(p, nullptr, 1, b->do_slope, b->do_d8, b->cpt, ctx->stream);
             if (e == hipSuccess) e = hipEventRecord(ev[2 * t + 1], ctx->stream);
         }

@@ -29,24 +29,35 @@
 //
 // Device formulation: the dependencies that matter (non-source inflows the
 // cell waits for) form a forest, each cell feeding at most its one target.
-// One 64-bit word per cell, kept in the output buffer, holds the cell's
-// static facts and an arrival accumulator:
+// A 16-bit static word per cell holds what a walk needs:
+//   bits  0-3   npend: inflows the cell waits for
+//   bits  4-7   nsrc: inflows from sources (each adds 1)
+//   bit   8     an inflow can never be available (border, or behind in
+//               the one-sweep order)
+//   bit   9     feeds: the cell is one of its target's pending inflows
+//   bits 10-13  the cell's code (window index of its target)
+//   bit  14     the cell has an area (interior, not NoData)
+// The static words sit in the context scratch in 8 x 8 blocks of 128 bytes,
+// one cache line each, so that the hops of a walk, which move to a
+// neighbour, mostly stay in a line that is already cached.
+// The output holds each cell's area as a double from the start when the
+// cell waits for nothing (1 + nsrc; NoData; 1 for a cell without a class).
+// A cell that waits (npend >= 1) holds an arrival accumulator instead until
+// it completes:
 //   bits  0-39  sum of the areas of done arrivals
 //   bits 40-43  arrivals that were not done
 //   bits 44-47  arrivals
-//   bits 48-51  npend: inflows the cell waits for
-//   bits 52-55  nsrc: inflows from sources (each adds 1)
-//   bit  56     an inflow can never be available (border, or behind in
-//               the one-sweep order)
-//   bit  57     feeds: the cell is one of its target's pending inflows
-//   bits 58-61  the cell's code (window index of its target)
-//   bit  62     the cell has an area (interior, not NoData)
-//   bit  63     the last-cell rule applies (value in the context scratch)
-// Every cell with npend == 0 that feeds starts a walk: it adds its result
-// to its target with one atomicAdd; the arrival that completes the target
-// (the returned word holds everything else) carries on from there.  Each
-// cell is finished exactly once, so the work is linear in the cells; the
-// walk is as long as the longest chain of completions.
+//   bit  63     the cell has not completed (a finished area is >= 1)
+// Every cell with npend == 0 that feeds starts a walk.  A hop reads the
+// target's static word: a target that waits for this walk alone completes
+// at once; a confluence (npend >= 2) takes an atomicAdd on its accumulator
+// and completes on the last arrival (the returned word holds the others).
+// The completing walk writes the target's area and carries on from there.
+// Each cell completes once, so the work is linear in the cells; the walk is
+// as long as the longest chain of completions.  Waiting cells and
+// completions are counted; cells that never complete (a cycle of codes,
+// which the converging mode admits for arbitrary codes) leave the counts
+// apart, and only then a last pass turns their accumulators into areas.
 #include <cmath>
 
 #include "internal.h"
@@ -58,6 +69,7 @@ namespace {
 constexpr unsigned kNone = 15;
 constexpr int kTileC = 64, kTileR = 32;  // cells per workgroup: 64 x 32
 constexpr uint64_t kAccMask = (1ull << 40) - 1;
+constexpr uint64_t kOpen = 1ull << 63;   // accumulator tag: not completed
 
 __device__ __forceinline__ unsigned flow_of(unsigned b) { return (b <= 8u && b != 4u) ? b : kNone; }
 __device__ __forceinline__ unsigned excl_of(unsigned b)
@@ -71,11 +83,30 @@ __device__ __forceinline__ bool is_valid(unsigned b) { return b != PARGO_FLOW_NO
 struct ScaParams {
     const uint8_t *__restrict__ flow;
     const uint32_t *__restrict__ mask;  // may be null
-    unsigned long long *__restrict__ w; // the output buffer, as words
+    unsigned long long *__restrict__ w; // the output buffer: areas (doubles) and accumulators
+    uint16_t *__restrict__ st;          // static words, 8 x 8 blocks
+    unsigned long long *__restrict__ counters;  // [0] walk cursor, [1] waiting cells, [2] completions
+    unsigned *__restrict__ last_done;   // done flags of the 3 x 3 cells ending at (rows-1, cols-1)
     long long pitch;
     int rows, cols, mask_words;
+    int bx, by;  // 8 x 8 blocks per block row, block rows
     int converge;
+    double nodata;
 };
+
+__device__ __forceinline__ unsigned npend_of(unsigned s) { return s & 15u; }
+__device__ __forceinline__ unsigned nsrc_of(unsigned s) { return (s >> 4) & 15u; }
+__device__ __forceinline__ bool poisoned_of(unsigned s) { return (s >> 8) & 1u; }
+__device__ __forceinline__ bool feeds_of(unsigned s) { return (s >> 9) & 1u; }
+__device__ __forceinline__ int code_of(unsigned s) { return (int)(s >> 10) & 15; }
+__device__ __forceinline__ bool area_of(unsigned s) { return (s >> 14) & 1u; }
+
+__device__ __forceinline__ long long st_index(const ScaParams &p, int r, int c)
+{
+    return ((long long)(r >> 3) * p.bx + (c >> 3)) * 64 + ((r & 7) << 3) + (c & 7);
+}
+
+__device__ __forceinline__ unsigned long long bits_of(double v) { return (unsigned long long)__double_as_longlong(v); }
 
 __device__ __forceinline__ unsigned load_class(const ScaParams &p, int r, int c)
 {
@@ -84,26 +115,67 @@ __device__ __forceinline__ unsigned load_class(const ScaParams &p, int r, int c)
     return p.flow[(long long)r * p.pitch + c];
 }
 
-// Static words of a 64 x 32 tile.  Classes of the tile plus a 2-cell halo
-// go to LDS, then the inflow mask of the tile plus a 1-cell halo (bit j set
-// when the neighbour at window position j < 4 ? j : j + 1 drains into the
-// cell and is not excluded; a cell needs to know which of its inflows are
-// sources, and whether its target counts it).
+__device__ __forceinline__ unsigned wave_sum(unsigned v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Counters and last-cell flags to zero, before the init kernel counts.
+__global__ void sca_reset_kernel(ScaParams p)
+{
+    const int i = threadIdx.x;
+    if (i < 3) p.counters[i] = 0;
+    if (i < 9) p.last_done[i] = 0;
+}
+
+// Static words and starting values of a 64 x 32 tile.  Classes of the tile
+// plus a 2-cell halo go to LDS, then the inflow mask of the tile plus a
+// 1-cell halo (bit j set when the neighbour at window position j < 4 ? j :
+// j + 1 drains into the cell and is not excluded; a cell needs to know which
+// of its inflows are sources, and whether its target counts it).  Tiles are
+// dealt to the 8 XCDs in contiguous runs (workgroup g runs on XCD g % 8), so
+// neighbouring tiles share their halo lines in one L2.
 __device__ __forceinline__ int win_of_bit(int j) { return j < 4 ? j : j + 1; }
 __device__ __forceinline__ int bit_of_win(int k) { return k < 4 ? k : k - 1; }
 
-__global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, unsigned long long *cursor)
+__global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x, int ntiles)
 {
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *cursor = 0;  // for the walk kernel
     constexpr int CW = kTileC + 4, CH = kTileR + 4;  // classes, halo 2
     constexpr int NW = kTileC + 2, NH = kTileR + 2;  // inflow masks, halo 1
     __shared__ uint8_t cls[CH][CW];
     __shared__ uint8_t inm[NH][NW];
+    __shared__ uint16_t stw[kTileR][kTileC];
+    __shared__ unsigned waiting[4];
     const int tid = threadIdx.x;
-    const int r0 = blockIdx.y * kTileR, c0 = blockIdx.x * kTileC;
-    for (int i = tid; i < CH * CW; i += 256) {
-        const int lr = i / CW, lc = i % CW;
-        cls[lr][lc] = (uint8_t)load_class(p, r0 - 2 + lr, c0 - 2 + lc);
+    const int g = blockIdx.x;
+    const int per = (ntiles + 7) / 8;  // tiles per XCD
+    const int tile = (g & 7) * per + (g >> 3);
+    if (tile >= ntiles) return;  // the whole workgroup
+    const int r0 = (tile / tiles_x) * kTileR, c0 = (tile % tiles_x) * kTileC;
+    {
+        // every load of the thread is issued before the first store to LDS,
+        // so that their latencies overlap
+        constexpr int PER = (CH * CW + 255) / 256;
+        unsigned v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * 256;
+            v[k] = PARGO_FLOW_NODATA;
+            if (i < CH * CW) {
+                const int r = r0 - 2 + i / CW, c = c0 - 2 + i % CW;
+                if (r >= 0 && r < p.rows && c >= 0 && c < p.cols) {
+                    const unsigned b = p.flow[(long long)r * p.pitch + c];
+                    const unsigned nd = p.mask ? (p.mask[(long long)r * p.mask_words + (c >> 5)] >> (c & 31)) & 1u : 0u;
+                    v[k] = nd ? (unsigned)PARGO_FLOW_NODATA : b;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * 256;
+            if (i < CH * CW) cls[i / CW][i % CW] = (uint8_t)v[k];
+        }
     }
     __syncthreads();
     for (int i = tid; i < NH * NW; i += 256) {
@@ -122,63 +194,75 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, unsigned lon
         inm[lr - 1][lc - 1] = (uint8_t)m;
     }
     __syncthreads();
+    unsigned nwait = 0;
     for (int i = tid; i < kTileR * kTileC; i += 256) {
         const int tr = i / kTileC, tc = i % kTileC;
         const int r = r0 + tr, c = c0 + tc;
-        if (r >= p.rows || c >= p.cols) continue;
-        const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
-        const unsigned b = cls[tr + 2][tc + 2];
-        unsigned long long word = 0;
-        if (interior && b == PARGO_FLOW_INERT) {
-            word = 1ull << 62;
-        } else if (interior && b != PARGO_FLOW_NODATA) {
-            const unsigned mine = inm[tr + 1][tc + 1];
-            unsigned npend = 0, nsrc = 0, poison = 0;
-            for (unsigned m = mine; m != 0; m &= m - 1) {
-                const int k = win_of_bit(__builtin_ctz(m));
-                const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
-                const bool u_interior = ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2;
-                if (!u_interior) poison = 1;
-                else if (inm[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0) nsrc++;
-                else if (p.converge || k < 4) npend++;
-                else poison = 1;
+        unsigned s = 0;
+        if (r < p.rows && c < p.cols) {
+            const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
+            const unsigned b = cls[tr + 2][tc + 2];
+            unsigned long long value = bits_of(p.nodata);
+            if (interior && b == PARGO_FLOW_INERT) {
+                s = 1u << 14;
+                value = bits_of(1.0);
+            } else if (interior && b != PARGO_FLOW_NODATA) {
+                const unsigned mine = inm[tr + 1][tc + 1];
+                unsigned npend = 0, nsrc = 0, poison = 0;
+                for (unsigned m = mine; m != 0; m &= m - 1) {
+                    const int k = win_of_bit(__builtin_ctz(m));
+                    const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
+                    const bool u_interior = ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2;
+                    if (!u_interior) poison = 1;
+                    else if (inm[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0) nsrc++;
+                    else if (p.converge || k < 4) npend++;
+                    else poison = 1;
+                }
+                const unsigned f = flow_of(b);
+                unsigned feeds = 0;
+                if (f != kNone && mine != 0 && (p.converge || f > 4)) {
+                    // the target counts this cell when its inflow mask has it
+                    const int trr = r + (int)f / 3 - 1, tcc = c + (int)f % 3 - 1;
+                    const bool t_interior = trr >= 1 && trr <= p.rows - 2 && tcc >= 1 && tcc <= p.cols - 2;
+                    const unsigned tm = inm[tr + 1 + (int)f / 3 - 1][tc + 1 + (int)f % 3 - 1];
+                    feeds = (t_interior && ((tm >> bit_of_win(8 - (int)f)) & 1u)) ? 1u : 0u;
+                }
+                s = npend | (nsrc << 4) | (poison << 8) | (feeds << 9) | ((f == kNone ? 0u : f) << 10) | (1u << 14);
+                value = npend ? kOpen : bits_of((double)(1u + nsrc));
+                nwait += npend ? 1u : 0u;
             }
-            const unsigned f = flow_of(b);
-            unsigned feeds = 0;
-            if (f != kNone && mine != 0 && (p.converge || f > 4)) {
-                // the target counts this cell when its inflow mask has it
-                const int trr = r + (int)f / 3 - 1, tcc = c + (int)f % 3 - 1;
-                const bool t_interior = trr >= 1 && trr <= p.rows - 2 && tcc >= 1 && tcc <= p.cols - 2;
-                const unsigned tm = inm[tr + 1 + (int)f / 3 - 1][tc + 1 + (int)f % 3 - 1];
-                feeds = (t_interior && ((tm >> bit_of_win(8 - (int)f)) & 1u)) ? 1u : 0u;
-            }
-            word = ((unsigned long long)npend << 48) | ((unsigned long long)nsrc << 52) |
-                   ((unsigned long long)poison << 56) | ((unsigned long long)feeds << 57) |
-                   ((unsigned long long)(f == kNone ? 0u : f) << 58) | (1ull << 62);
+            p.w[(long long)r * p.pitch + c] = value;
         }
-        p.w[(long long)r * p.pitch + c] = word;
+        stw[tr][tc] = (uint16_t)s;
+    }
+    nwait = wave_sum(nwait);
+    if ((tid & 63) == 0) waiting[tid >> 6] = nwait;
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned n = waiting[0] + waiting[1] + waiting[2] + waiting[3];
+        if (n) atomicAdd(&p.counters[1], (unsigned long long)n);
+    }
+    // the tile's 4 x 8 blocks, one wave-wide 128-byte line per block
+    for (int i = tid; i < kTileR * kTileC; i += 256) {
+        const int blk = i >> 6, within = i & 63;
+        const int tr = (blk >> 3) * 8 + (within >> 3), tc = (blk & 7) * 8 + (within & 7);
+        const int r = r0 + tr, c = c0 + tc;
+        if ((r >> 3) < p.by && (c >> 3) < p.bx) p.st[st_index(p, r, c)] = stw[tr][tc];
     }
 }
 
-__device__ __forceinline__ unsigned npend_of(unsigned long long w) { return (unsigned)(w >> 48) & 15u; }
-__device__ __forceinline__ unsigned nsrc_of(unsigned long long w) { return (unsigned)(w >> 52) & 15u; }
-__device__ __forceinline__ bool poisoned_of(unsigned long long w) { return (w >> 56) & 1ull; }
-__device__ __forceinline__ bool feeds_of(unsigned long long w) { return (w >> 57) & 1ull; }
-__device__ __forceinline__ int code_of(unsigned long long w) { return (int)(w >> 58) & 15; }
-
 // Walks, persistent: every lane runs one hop per iteration; a lane without
-// a walk takes the next cell of its wave's chunk (chunks of kChunk cells come
-// from a global cursor) and starts a walk there if the cell waits for
-// nothing and feeds its target.  Lanes stay busy until the cursor passes the
-// last cell and every walk of the wave has ended.  Checking one cell per
-// idle lane and iteration keeps the number of hops in flight, and so the
-// queueing delay on the longest chain, moderate: starting every walk at
-// once was slower.
+// a walk takes the next cell of its wave's chunk (chunks of kChunk cells in
+// block order come from a global cursor) and starts a walk there if the cell
+// waits for nothing and feeds its target.  Lanes stay busy until the cursor
+// passes the last cell and every walk of the wave has ended.  Checking one
+// cell per idle lane and iteration keeps the number of hops in flight, and
+// so the queueing delay on the longest chain, moderate: starting every walk
+// at once was slower.
 //
-// A hop first reads the target's word.  A target with one pending inflow
-// has this walk as its only writer, so the walk completes it with a plain
-// store; only confluences (two or more pending inflows, a few percent of
-// the hops on D8 grids) take the atomicAdd.
+// The static words do not change during the walk, so their loads may hit
+// any cache.  Accumulators are only touched by atomics until their cell
+// completes; after that only the completing lane writes the area there.
 constexpr long long kChunk = 4096;
 
 __device__ __forceinline__ long long first_lane(long long v)
@@ -188,37 +272,39 @@ __device__ __forceinline__ long long first_lane(long long v)
     return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
-__global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p, unsigned long long *cursor)
+__global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p)
 {
-    const long long ncells = (long long)p.rows * p.cols;
+    const long long nwords = (long long)p.by * p.bx * 64;
     long long chunk_next = 0, chunk_end = 0;  // the same in every lane of a wave
     bool more = true;                         // the same in every lane of a wave
-    long long cur = -1;                       // cell this lane's walk stands on
+    int cr = -1, cc = 0;                      // cell this lane's walk stands on (cr < 0: none)
     int code = 0;
     unsigned long long val = 0;
     bool done = false;
+    unsigned completions = 0;
     for (;;) {
-        // the target's word is requested first, so that its latency
+        // the target's static word is requested first, so that its latency
         // overlaps the refill below
-        long long t = -1;
-        unsigned long long tw = 0;
-        if (cur >= 0) {
-            t = cur + (long long)(code / 3 - 1) * p.pitch + (code % 3 - 1);
-            tw = p.w[t];
+        int tr = 0, tc = 0;
+        unsigned ts = 0;
+        if (cr >= 0) {
+            tr = cr + code / 3 - 1;
+            tc = cc + code % 3 - 1;
+            ts = p.st[st_index(p, tr, tc)];
         }
-        const unsigned long long need = __ballot(cur < 0);
-        long long fresh = -1;
-        unsigned long long fw = 0;
+        const unsigned long long need = __ballot(cr < 0);
+        int fr = -1, fc = 0;
+        unsigned fs = 0;
         if (need != 0 && more) {
             if (chunk_next >= chunk_end) {
                 long long b = 0;
-                if (__lane_id() == 0) b = (long long)atomicAdd(cursor, (unsigned long long)kChunk);
+                if (__lane_id() == 0) b = (long long)atomicAdd(&p.counters[0], (unsigned long long)kChunk);
                 b = first_lane(b);
-                if (b >= ncells) {
+                if (b >= nwords) {
                     more = false;
                 } else {
                     chunk_next = b;
-                    chunk_end = b + kChunk < ncells ? b + kChunk : ncells;
+                    chunk_end = b + kChunk < nwords ? b + kChunk : nwords;
                 }
             }
             if (more) {
@@ -226,48 +312,73 @@ __global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p, unsigned lon
                     __builtin_amdgcn_mbcnt_hi((unsigned)(need >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)need, 0u));
                 const long long npop = __popcll(need);
                 const long long take = npop < chunk_end - chunk_next ? npop : chunk_end - chunk_next;
-                if (cur < 0 && (long long)rank < take) {
+                if (cr < 0 && (long long)rank < take) {
                     const long long idx = chunk_next + rank;
-                    const int r = (int)(idx / p.cols), c = (int)(idx - (long long)r * p.cols);
-                    fresh = (long long)r * p.pitch + c;
-                    fw = p.w[fresh];
+                    fs = p.st[idx];
+                    const long long blk = idx >> 6;
+                    const int within = (int)(idx & 63);
+                    const int br = (int)(blk / p.bx);
+                    fr = br * 8 + (within >> 3);
+                    fc = (int)(blk - (long long)br * p.bx) * 8 + (within & 7);
                 }
                 chunk_next += take;
             }
         }
-        if (cur >= 0) {
-            const unsigned long long add = (done ? val : 0ull) | ((unsigned long long)(!done) << 40) | (1ull << 44);
-            unsigned long long old;
-            bool complete;
-            if (npend_of(tw) == 1) {
-                old = tw;  // nobody else writes this word
-                p.w[t] = tw + add;
-                complete = true;
-            } else {
+        if (cr >= 0) {
+            const long long t = (long long)tr * p.pitch + tc;
+            bool complete = true;
+            unsigned long long old = 0;  // other arrivals: none when the walk is the only one awaited
+            if (npend_of(ts) != 1) {
+                const unsigned long long add =
+                    (done ? val : 0ull) | ((unsigned long long)(!done) << 40) | (1ull << 44);
                 old = atomicAdd(&p.w[t], add);
-                complete = ((unsigned)(old >> 44) & 15u) + 1u == npend_of(old);
+                complete = ((unsigned)(old >> 44) & 15u) + 1u == npend_of(ts);
             }
             if (!complete) {
-                cur = -1;  // an inflow is still on its way
+                cr = -1;  // an inflow is still on its way
             } else {
                 const unsigned long long acc = (old & kAccMask) + (done ? val : 0ull);
                 const unsigned not_done = ((unsigned)(old >> 40) & 15u) + (done ? 0u : 1u);
-                done = !poisoned_of(old) && not_done == 0;
-                val = 1ull + nsrc_of(old) + acc;
-                if (feeds_of(old)) {
-                    cur = t;
-                    code = code_of(old);
+                done = !poisoned_of(ts) && not_done == 0;
+                val = 1ull + nsrc_of(ts) + acc;
+                p.w[t] = bits_of((double)val);
+                completions++;
+                if (tr >= p.rows - 3 && tc >= p.cols - 3)
+                    p.last_done[(tr - (p.rows - 3)) * 3 + (tc - (p.cols - 3))] = done ? 1u : 0u;
+                if (feeds_of(ts)) {
+                    cr = tr;
+                    cc = tc;
+                    code = code_of(ts);
                 } else {
-                    cur = -1;
+                    cr = -1;
                 }
             }
-        } else if (fresh >= 0 && ((fw >> 62) & 1ull) && npend_of(fw) == 0 && feeds_of(fw)) {
-            cur = fresh;
-            code = code_of(fw);
-            val = 1ull + nsrc_of(fw);
-            done = !poisoned_of(fw);
+        } else if (fr >= 0 && area_of(fs) && npend_of(fs) == 0 && feeds_of(fs)) {
+            cr = fr;
+            cc = fc;
+            code = code_of(fs);
+            val = 1ull + nsrc_of(fs);
+            done = !poisoned_of(fs);
         }
-        if (!more && __ballot(cur >= 0) == 0) break;
+        if (!more && __ballot(cr >= 0) == 0) break;
+    }
+    completions = wave_sum(completions);
+    if (__lane_id() == 0 && completions) atomicAdd(&p.counters[2], (unsigned long long)completions);
+}
+
+// Cells that never completed (their counts disagree only then): the area of
+// the done arrivals they received, as the converging semantics defines it.
+__global__ __launch_bounds__(256) void sca_open_kernel(ScaParams p)
+{
+    if (p.counters[1] == p.counters[2]) return;
+    const long long n = (long long)p.rows * p.cols;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int r = (int)(i / p.cols), c = (int)(i - (long long)r * p.cols);
+        const unsigned s = p.st[st_index(p, r, c)];
+        if (!area_of(s) || npend_of(s) == 0) continue;
+        const long long o = (long long)r * p.pitch + c;
+        const unsigned long long w = p.w[o];
+        if (w & kOpen) p.w[o] = bits_of((double)(1ull + nsrc_of(s) + (w & kAccMask)));
     }
 }
 
@@ -275,37 +386,23 @@ __global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p, unsigned lon
 // loop (scaOperator.cpp:86-108).  When it is NoData (or a cell that takes no
 // inflow) it adds, in window order, the area of every interior inflowing
 // neighbour that is done; all of those precede it in row-major order.
-__global__ void sca_last_cell_kernel(ScaParams p, double nodata, double *slot)
+__global__ void sca_last_cell_kernel(ScaParams p)
 {
     const int r = p.rows - 2, c = p.cols - 2;
     const unsigned b = load_class(p, r, c);
     if (is_valid(b)) return;
-    double v = b == PARGO_FLOW_NODATA ? nodata : 1.0;
+    double v = b == PARGO_FLOW_NODATA ? p.nodata : 1.0;
     for (int k = 0; k < 9; k++) {
         if (k == 4) continue;
         const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
         if (ur < 1 || ur > p.rows - 2 || uc < 1 || uc > p.cols - 2) continue;
         if (flow_of(load_class(p, ur, uc)) != (unsigned)(8 - k)) continue;
-        const unsigned long long w = p.w[(long long)ur * p.pitch + uc];
-        const bool done = !poisoned_of(w) && ((w >> 40) & 15ull) == 0 && ((w >> 44) & 15ull) == npend_of(w);
-        if (done) v += (double)(1ull + nsrc_of(w) + (w & kAccMask));
+        const unsigned s = p.st[st_index(p, ur, uc)];
+        const bool done = npend_of(s) == 0 ? !poisoned_of(s)
+                                           : p.last_done[(ur - (p.rows - 3)) * 3 + (uc - (p.cols - 3))] != 0;
+        if (done) v += __longlong_as_double((long long)p.w[(long long)ur * p.pitch + uc]);
     }
-    slot[0] = v;
-    p.w[(long long)r * p.pitch + c] |= 1ull << 63;
-}
-
-// Words to areas, in place.
-__global__ __launch_bounds__(256) void sca_final_kernel(ScaParams p, double nodata, const double *slot)
-{
-    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
-    if (c >= p.cols) return;
-    const long long i = (long long)r * p.pitch + c;
-    const unsigned long long w = p.w[i];
-    double v;
-    if (w >> 63) v = slot[0];
-    else if ((w >> 62) & 1ull) v = (double)(1ull + nsrc_of(w) + (w & kAccMask));
-    else v = nodata;
-    reinterpret_cast<double *>(p.w)[i] = v;
+    p.w[(long long)r * p.pitch + c] = bits_of(v);
 }
 
 // Reference D8 layer (double) to flow classes: NoData when
@@ -396,29 +493,40 @@ extern "C" int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *g, const uint8_t *
     int rc = pargo::bind(ctx);
     if (rc) return rc;
     const bool last_rule = mode == PARGO_SCA_REFERENCE && g->rows >= 3 && g->cols >= 3;
-    // The final kernel's read of the slot is guarded per cell, but the
-    // compiler issues it as a scalar load that runs whatever the lanes
-    // decide, so the slot must be a valid address in every mode.
-    rc = pargo::ensure_scratch(ctx, 2 * sizeof(double));
+    const int tiles_x = (g->cols + pargo::kTileC - 1) / pargo::kTileC;
+    const long long ntiles = (long long)tiles_x * ((g->rows + pargo::kTileR - 1) / pargo::kTileR);
+    if (ntiles > (1ll << 24))
+        return fail(ctx, PARGO_E_INVALID, "grid of %d x %d cells needs %lld tiles, more than 2^24", g->rows, g->cols,
+                    ntiles);
+    // scratch: counters and last-cell flags, then the static words in 8 x 8
+    // blocks (2 bytes per cell, rows and columns rounded up to 8)
+    const int bx = (g->cols + 7) / 8, by = (g->rows + 7) / 8;
+    const size_t head = 256;
+    rc = pargo::ensure_scratch(ctx, head + (size_t)bx * by * 64 * sizeof(uint16_t));
     if (rc) return rc;
-    double *slot = static_cast<double *>(ctx->scratch);
-    unsigned long long *cursor = reinterpret_cast<unsigned long long *>(slot + 1);
+    char *scr = static_cast<char *>(ctx->scratch);
     pargo::ScaParams p{};
     p.flow = flow;
     p.mask = nodata_mask;
     p.w = reinterpret_cast<unsigned long long *>(sca_out);
+    p.st = reinterpret_cast<uint16_t *>(scr + head);
+    p.counters = reinterpret_cast<unsigned long long *>(scr);
+    p.last_done = reinterpret_cast<unsigned *>(scr + 64);
     p.pitch = pitch;
     p.rows = g->rows;
     p.cols = g->cols;
     p.mask_words = (g->cols + 31) / 32;
+    p.bx = bx;
+    p.by = by;
     p.converge = mode == PARGO_SCA_CONVERGE;
-    dim3 tiles((g->cols + pargo::kTileC - 1) / pargo::kTileC, (g->rows + pargo::kTileR - 1) / pargo::kTileR);
-    hipLaunchKernelGGL(pargo::sca_init_kernel, tiles, dim3(256), 0, ctx->stream, p, cursor);
+    p.nodata = g->nodata;
+    hipLaunchKernelGGL(pargo::sca_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, p);
+    const int per_xcd = (int)((ntiles + 7) / 8);
+    hipLaunchKernelGGL(pargo::sca_init_kernel, dim3(8 * per_xcd), dim3(256), 0, ctx->stream, p, tiles_x, (int)ntiles);
     // persistent walk: 8 workgroups of 4 waves per CU
     const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
-    hipLaunchKernelGGL(pargo::sca_walk_kernel, dim3(cus * 8), dim3(256), 0, ctx->stream, p, cursor);
-    dim3 rowsgrid((g->cols + 255) / 256, g->rows);
-    if (last_rule) hipLaunchKernelGGL(pargo::sca_last_cell_kernel, dim3(1), dim3(1), 0, ctx->stream, p, g->nodata, slot);
-    hipLaunchKernelGGL(pargo::sca_final_kernel, rowsgrid, dim3(256), 0, ctx->stream, p, g->nodata, slot);
+    hipLaunchKernelGGL(pargo::sca_walk_kernel, dim3(cus * 8), dim3(256), 0, ctx->stream, p);
+    hipLaunchKernelGGL(pargo::sca_open_kernel, dim3(cus * 4), dim3(256), 0, ctx->stream, p);
+    if (last_rule) hipLaunchKernelGGL(pargo::sca_last_cell_kernel, dim3(1), dim3(1), 0, ctx->stream, p);
     return pargo::check_launch(ctx, "pargo_sca_d8");
 }
