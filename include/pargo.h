@@ -244,7 +244,9 @@ int pargo_d8_classify(pargo_ctx *ctx, const pargo_grid *grid, const double *d8, 
  * result bit for bit, including that it stops after one accumulation sweep
  * in row-major order and that its last work cell (rows-2, cols-2) adds
  * inflows to NoData; PARGO_SCA_CONVERGE gives the area that repeated sweeps
- * converge to (full D8 flow accumulation).  grid->cellsize is unused. */
+ * converge to (full D8 flow accumulation).  grid->cellsize is unused.
+ * Working memory: the context's scratch grows to 2 bytes per cell (rows and
+ * columns rounded up to 8) plus 256 bytes, kept until pargo_destroy. */
 int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *grid, const uint8_t *flow, const uint32_t *nodata_mask,
                  int mode, double *sca_out);
 
@@ -262,7 +264,9 @@ int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *grid, const uint8_t *flow, co
  * [min(dem, 10000), 10000]) is refused with PARGO_E_UNSUPPORTED, as is a
  * DEM holding NaN (the reference never settles next to NoData there).
  * Synchronous.  *passes (host, may be NULL) receives the number of device
- * passes. */
+ * passes.  Working memory: the context's scratch grows to 524 bytes per
+ * 64 x 32 tile (worklists, stamps and a copy of each tile's edge columns),
+ * kept until pargo_destroy. */
 int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *grid, const float *dem, double *filled_out, int *passes);
 
 /* ---- IDW ------------------------------------------------------------ */

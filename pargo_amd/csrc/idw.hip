@@ -129,6 +129,10 @@ struct IdwParams {
 constexpr double kIdwEps = 0.0000001;  // idwOperator.h:13
 constexpr double kPrefilterSlack = 1.0 + 0x1p-40;
 
+// The reference's distance from its squared form (idwOperator.cpp:398-402):
+// the square root, or 1e-7 for a coincident sample.
+__device__ __forceinline__ double idw_dist(double s) { return s == 0.0 ? kIdwEps : sqrt(s); }
+
 // rowAtOtherLayer: (int)min(scale * idx, other - 1) with double scale.
 __device__ __forceinline__ int mask_index(int other, int self, int idx)
 {
@@ -216,7 +220,7 @@ __device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow
         if (pre <= kthr) {
             const double dx = xy.x - cellX, dy = xy.y - cellY;
             const double s = dx * dx + dy * dy;
-            const double tmpDist = (s == 0.0) ? kIdwEps : sqrt(s);
+            const double tmpDist = idw_dist(s);
             if (tail < K - 1) {
                 tail++;
                 kd[tail * 256] = tmpDist;
@@ -322,6 +326,16 @@ __device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow
     return (tail + 1 < K) ? tail + 1 : K;
 }
 
+// v of lane l (l wave-uniform) in every lane: two v_readlane_b32 into
+// scalar registers, where a shuffle would take the LDS crossbar twice
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // The same search with each ring's candidates visited nearest-first, for
 // the lanes with `mine` set.  The reference's result is the K smallest
 // distances among the samples of the visited rings, whatever order they are
@@ -374,7 +388,7 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
         if (pre <= kthr) {
             const double dx = xy.x - cellX, dy = xy.y - cellY;
             const double s = dx * dx + dy * dy;
-            const double tmpDist = (s == 0.0) ? kIdwEps : sqrt(s);
+            const double tmpDist = idw_dist(s);
             if (tail < K - 1) {
                 tail++;
                 kd[tail * 256] = tmpDist;
@@ -425,7 +439,7 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
                 while (m) {
                     const int l = (int)__builtin_ctzll(m);
                     m &= m - 1;
-                    const double2 xy = make_double2(__shfl(sx[k], l, 64), __shfl(sy[k], l, 64));
+                    const double2 xy = make_double2(lane_value(sx[k], l), lane_value(sy[k], l));
                     const int q = __builtin_amdgcn_readlane(sqb[k], l) >> kBinBits;
                     const double dx = xy.x - cellX, dy = xy.y - cellY;
                     visit(xy, __builtin_fma(dx, dx, dy * dy), q);
@@ -574,17 +588,28 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
             if (tie) num = num2;
         }
         if (mine) {
-            // weights in slot order (idwOperator.cpp:472-480)
+            // weights 1 / d^p summed in slot order (idwOperator.cpp:472-480).
+            // The reference forms each weight twice and divides every term
+            // by the sum; here each weight is formed once (kept in the
+            // list's distance slot) and the weighted sum divided once, which
+            // moves the result by a few FP64 ulps, far inside the float
+            // output's rounding.
             double wsum = 0.0;
-            for (int i = 0; i < num; i++) wsum += 1.0 / powi(kd[i * 256], p.power);
+            for (int i = 0; i < num; i++) {
+                const double wi = 1.0 / powi(kd[i * 256], p.power);
+                kd[i * 256] = wi;
+                wsum += wi;
+            }
             if (ADJOINT) {
                 const double gc = (double)p.grad_out[(long long)row * p.cols + col];
-                if (gc != 0.0)
-                    for (int i = 0; i < num; i++) adj_add(ks[i * 256], gc * (1.0 / powi(kd[i * 256], p.power)) / wsum);
+                if (gc != 0.0 && num > 0) {
+                    const double scale = gc / wsum;
+                    for (int i = 0; i < num; i++) adj_add(ks[i * 256], kd[i * 256] * scale);
+                }
             } else {
                 double acc = 0.0;
-                for (int i = 0; i < num; i++) acc += p.v[ks[i * 256]] * (1.0 / powi(kd[i * 256], p.power)) / wsum;
-                result = (float)acc;
+                for (int i = 0; i < num; i++) acc += p.v[ks[i * 256]] * kd[i * 256];
+                result = num > 0 ? (float)(acc / wsum) : 0.0f;  // no sample in range: the empty sum, 0
             }
         }
         pending = pending && !mine;

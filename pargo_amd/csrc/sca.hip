@@ -208,16 +208,24 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
                 value = bits_of(1.0);
             } else if (interior && b != PARGO_FLOW_NODATA) {
                 const unsigned mine = inm[tr + 1][tc + 1];
-                unsigned npend = 0, nsrc = 0, poison = 0;
-                for (unsigned m = mine; m != 0; m &= m - 1) {
-                    const int k = win_of_bit(__builtin_ctz(m));
+                // per inflow bit j: the neighbour is interior (intn), is a
+                // source (srcn: nothing drains into it), may be waited for
+                // (it precedes the cell in the one-sweep order, or any in
+                // the converging mode).  Eight unconditional reads and mask
+                // arithmetic, so that no wave loops over its busiest cell.
+                unsigned srcn = 0, intn = 0;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int k = win_of_bit(j);
                     const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
-                    const bool u_interior = ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2;
-                    if (!u_interior) poison = 1;
-                    else if (inm[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0) nsrc++;
-                    else if (p.converge || k < 4) npend++;
-                    else poison = 1;
+                    srcn |= (inm[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0 ? 1u : 0u) << j;
+                    intn |= (ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2 ? 1u : 0u) << j;
                 }
+                const unsigned waitable = p.converge ? 0xFFu : 0x0Fu;  // window bits 0-3: NW, N, NE, W
+                const unsigned in_int = mine & intn;
+                const unsigned nsrc = __builtin_popcount(in_int & srcn);
+                const unsigned npend = __builtin_popcount(in_int & ~srcn & waitable);
+                const unsigned poison = ((mine & ~intn) | (in_int & ~srcn & ~waitable)) != 0 ? 1u : 0u;
                 const unsigned f = flow_of(b);
                 unsigned feeds = 0;
                 if (f != kNone && mine != 0 && (p.converge || f > 4)) {
