@@ -480,6 +480,8 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
                     const int take = min(s1 - q0, 64 * kStage - staged);
 #pragma unroll
                     for (int k = 0; k < kStage; k++) {
+                        // registers the block's slots do not reach (wave-uniform)
+                        if (64 * k + 64 <= staged || 64 * k >= staged + take) continue;
                         const int i = 64 * k + lane;
                         if (i >= staged && i < staged + take) {
                             const int q = q0 + (i - staged);

@@ -144,6 +144,7 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
     constexpr int CW = kTileC + 4, CH = kTileR + 4;  // classes, halo 2
     constexpr int NW = kTileC + 2, NH = kTileR + 2;  // inflow masks, halo 1
     __shared__ uint8_t cls[CH][CW];
+    __shared__ uint8_t tgt[CH][CW];  // flow_of(class): window index of the target, kNone if none
     __shared__ uint8_t inm[NH][NW];
     __shared__ uint16_t stw[kTileR][kTileC];
     __shared__ unsigned waiting[4];
@@ -174,7 +175,10 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int i = tid + k * 256;
-            if (i < CH * CW) cls[i / CW][i % CW] = (uint8_t)v[k];
+            if (i < CH * CW) {
+                cls[i / CW][i % CW] = (uint8_t)v[k];
+                tgt[i / CW][i % CW] = (uint8_t)flow_of(v[k]);
+            }
         }
     }
     __syncthreads();
@@ -183,13 +187,14 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
         const unsigned b = cls[lr][lc];
         unsigned m = 0;
         if (is_valid(b)) {
-            const unsigned ex = excl_of(b);
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int k = win_of_bit(j);
-                const unsigned u = cls[lr + k / 3 - 1][lc + k % 3 - 1];
-                m |= (flow_of(u) == (unsigned)(8 - k) && ex != (unsigned)k) ? 1u << j : 0u;
+                m |= (tgt[lr + k / 3 - 1][lc + k % 3 - 1] == (unsigned)(8 - k)) ? 1u << j : 0u;
             }
+            // the excluded window position (at most one) takes no inflow
+            const unsigned ex = excl_of(b);
+            if (ex != kNone && ex != 4u) m &= ~(1u << bit_of_win((int)ex));
         }
         inm[lr - 1][lc - 1] = (uint8_t)m;
     }
@@ -213,13 +218,21 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
                 // (it precedes the cell in the one-sweep order, or any in
                 // the converging mode).  Eight unconditional reads and mask
                 // arithmetic, so that no wave loops over its busiest cell.
-                unsigned srcn = 0, intn = 0;
+                unsigned srcn = 0, intn = 0xFFu;
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const int k = win_of_bit(j);
-                    const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
                     srcn |= (inm[tr + 1 + k / 3 - 1][tc + 1 + k % 3 - 1] == 0 ? 1u : 0u) << j;
-                    intn |= (ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2 ? 1u : 0u) << j;
+                }
+                // only a cell next to the border has neighbours outside the interior
+                if (r <= 1 || r >= p.rows - 2 || c <= 1 || c >= p.cols - 2) {
+                    intn = 0;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int k = win_of_bit(j);
+                        const int ur = r + k / 3 - 1, uc = c + k % 3 - 1;
+                        intn |= (ur >= 1 && ur <= p.rows - 2 && uc >= 1 && uc <= p.cols - 2 ? 1u : 0u) << j;
+                    }
                 }
                 const unsigned waitable = p.converge ? 0xFFu : 0x0Fu;  // window bits 0-3: NW, N, NE, W
                 const unsigned in_int = mine & intn;
