@@ -66,6 +66,23 @@ struct FillParams {
 
 __device__ __forceinline__ bool is_nd(const FillParams &p, float v) { return v >= p.nd_lo && v <= p.nd_hi; }
 
+// Lane i gets lane i-1's (from_left) or lane i+1's (from_right) v; the lane
+// without a source (0, resp. 63) gets `edge`.  Two 32-bit DPP wave shifts.
+__device__ __forceinline__ double dpp_from_left(double v, double edge)
+{
+    const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)b, 0x138, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(e >> 32), (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double dpp_from_right(double v, double edge)
+{
+    const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)b, 0x130, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(e >> 32), (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Each tile's first and last column of W, also kept contiguously per tile:
 // a tile's halo columns are its neighbours' edge columns, read as two short
 // runs instead of one extra cache line per halo row on each side.
@@ -205,51 +222,67 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
             if (n != 4) any |= fabs(sw[lr + n / 3 - 1][lc + n % 3 - 1] - p.nodata) < 0.000001;
         nd_nbr |= (unsigned)any << k;
     }
+    // The wave's 8 cells per lane live in registers during the sweeps; left
+    // and right neighbours come from the adjacent lanes by DPP, the halo
+    // columns (lane 0: the left one, lane 63: the right one) from LDS, and
+    // the rows above and below the wave's band from LDS, where the waves
+    // publish their first and last rows after every half-sweep.
+    const int lane = tid & 63;
+    double wv[kRW];
+#pragma unroll
+    for (int k = 0; k < kRW; k++) wv[k] = sw[rbase + k + 1][lc];
+    // halo column of tile row j-1 (j = sw row): the left one for lane 0, the
+    // right one for lane 63 (static during the visit; a broadcast LDS read)
+    const int ecol = lane == 0 ? 0 : kTW + 1;
+    auto eg = [&](int j) -> double { return sw[rbase + j][ecol]; };
+    double up_b = sw[rbase][lc], dn_b = sw[rbase + kRW + 1][lc];
     unsigned touched = 0;  // bit k: the cell changed during this pass
     // One update of the cell in tile row rbase + k; true if it changed.
     // Without a NoData neighbour the reference's window loop gives
     // W = dem if dem >= W(n) + gap for some n, else min(W, min_n W(n) + gap);
     // the rounded addition is monotone, so min_n (W(n) + gap) is
     // (min_n W(n)) + gap with one rounding, and "for some n" is a test
-    // against that minimum.
+    // against that minimum.  Every lane runs every step (the DPP moves read
+    // the neighbouring lanes), and the cells that must not change keep
+    // their value by selection.
     auto update = [&](int k) -> bool {
-        if (!((live >> k) & 1u)) return false;
-        const int lr = rbase + k + 1;
-        const double w = sw[lr][lc];
+        const double up = k == 0 ? up_b : wv[k > 0 ? k - 1 : 0];
+        const double dn = k == kRW - 1 ? dn_b : wv[k < kRW - 1 ? k + 1 : 0];
+        const double w = wv[k];
+        double wmin = fmin(fmin(dpp_from_left(up, eg(k)), up), dpp_from_right(up, eg(k)));
+        wmin = fmin(wmin, fmin(dpp_from_left(w, eg(k + 1)), dpp_from_right(w, eg(k + 1))));
+        wmin = fmin(wmin, fmin(fmin(dpp_from_left(dn, eg(k + 2)), dn), dpp_from_right(dn, eg(k + 2))));
         const double dk = (double)dem_f[k];
-        if (w <= dk) return false;
-        double nw = dk;
-        if (!((nd_nbr >> k) & 1u)) {
-            double wmin = sw[lr - 1][lc - 1];
-#pragma unroll
-            for (int n = 1; n < 9; n++)
-                if (n != 4) wmin = fmin(wmin, sw[lr + n / 3 - 1][lc + n % 3 - 1]);
-            const double sm = wmin + kGap;
-            nw = dk >= sm ? dk : fmin(w, sm);
-        }
-        if (nw == w) return false;
-        sw[lr][lc] = nw;
-        touched |= 1u << k;
-        return true;
+        const double sm = wmin + kGap;
+        const double nw = ((nd_nbr >> k) & 1u) ? dk : (dk >= sm ? dk : fmin(w, sm));
+        const bool ch = ((live >> k) & 1u) && w > dk && nw != w;
+        wv[k] = ch ? nw : w;
+        touched |= (unsigned)ch << k;
+        return ch;
     };
     // Each iteration sweeps the wave's 8 rows down and then up, one row per
-    // step: a lowered value travels 8 rows and up to 16 columns per
-    // iteration.  The wave-scope fences order the LDS writes of one step
-    // before the reads of the next (any order would still converge to the
-    // same values; the fences only make the sweeps effective).
+    // step (a lowered value travels 8 rows and up to 16 columns), with the
+    // band edges exchanged between the half-sweeps.  A wave may read a
+    // neighbour's row before or after that neighbour's latest change; either
+    // feeds the monotone iteration a value at or above the fixed point, and
+    // the loop ends only after an iteration in which no cell changed.
     for (;;) {
         bool any = false;
 #pragma unroll
-        for (int k = 0; k < kRW; k++) {
-            any |= update(k);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        }
+        for (int k = 0; k < kRW; k++) any |= update(k);
+        sw[rbase + 1][lc] = wv[0];
+        sw[rbase + kRW][lc] = wv[kRW - 1];
+        __syncthreads();
+        up_b = sw[rbase][lc];
+        dn_b = sw[rbase + kRW + 1][lc];
 #pragma unroll
-        for (int k = kRW - 1; k >= 0; k--) {
-            any |= update(k);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        }
-        if (!__syncthreads_or(any)) break;
+        for (int k = kRW - 1; k >= 0; k--) any |= update(k);
+        sw[rbase + 1][lc] = wv[0];
+        sw[rbase + kRW][lc] = wv[kRW - 1];
+        const bool more = __syncthreads_or(any);
+        up_b = sw[rbase][lc];
+        dn_b = sw[rbase + kRW + 1][lc];
+        if (!more) break;
     }
     unsigned flags = 0;
 #pragma unroll
@@ -257,7 +290,7 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
         if (!((touched >> k) & 1u)) continue;
         const int tr = rbase + k, tc = lc - 1;
         const int r = r0 + tr, c = c0 + tc;
-        const double nw = sw[tr + 1][lc];
+        const double nw = wv[k];
         p.w[(long long)r * p.pitch + c] = nw;
         if (tc == 0 || tc == kTW - 1) p.edge[edge_index(p, ty, tx, tc != 0, tr)] = nw;
         const unsigned top = tr == 0, bottom = tr == kTH - 1, left = tc == 0, right = tc == kTW - 1;
@@ -288,7 +321,7 @@ __device__ __forceinline__ void fill_tile(const FillParams &p, int tile, int pas
 }
 
 
-__global__ __launch_bounds__(256, 5) void fill_pass_kernel(FillParams p, int pass, int batch_first)
+__global__ __launch_bounds__(256, 4) void fill_pass_kernel(FillParams p, int pass, int batch_first)
 {
     __shared__ double sw[kTH + 2][kTW + 2];
     __shared__ int edge_flags, run;
