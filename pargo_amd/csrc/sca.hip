@@ -544,9 +544,12 @@ extern "C" int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *g, const uint8_t *
     hipLaunchKernelGGL(pargo::sca_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, p);
     const int per_xcd = (int)((ntiles + 7) / 8);
     hipLaunchKernelGGL(pargo::sca_init_kernel, dim3(8 * per_xcd), dim3(256), 0, ctx->stream, p, tiles_x, (int)ntiles);
-    // persistent walk: 8 workgroups of 4 waves per CU
+    // persistent walk: 4 workgroups of 4 waves per CU.  Fewer walks in flight
+    // queue less on the longest chains; measured at 16384^2 (reference /
+    // converging mode): 1 per CU 8.0 / 11.6 ms, 2 5.4 / 8.3, 4 4.4 / 7.6,
+    // 8 4.6 / 8.3, 16 4.6 / 8.3
     const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
-    hipLaunchKernelGGL(pargo::sca_walk_kernel, dim3(cus * 8), dim3(256), 0, ctx->stream, p);
+    hipLaunchKernelGGL(pargo::sca_walk_kernel, dim3(cus * 4), dim3(256), 0, ctx->stream, p);
     hipLaunchKernelGGL(pargo::sca_open_kernel, dim3(cus * 4), dim3(256), 0, ctx->stream, p);
     if (last_rule) hipLaunchKernelGGL(pargo::sca_last_cell_kernel, dim3(1), dim3(1), 0, ctx->stream, p);
     return pargo::check_launch(ctx, "pargo_sca_d8");
