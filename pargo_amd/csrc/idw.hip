@@ -129,6 +129,17 @@ struct IdwParams {
 constexpr double kIdwEps = 0.0000001;  // idwOperator.h:13
 constexpr double kPrefilterSlack = 1.0 + 0x1p-40;
 
+// A load at a wave-uniform address of data no kernel of this launch writes
+// (the sample buckets): through the constant address space the compiler
+// can issue it as a scalar load, which the block walk waits on far less
+// than on a vector load per block.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T *ptr)
+{
+    typedef const __attribute__((address_space(4))) T constant_t;
+    return *(constant_t *)(unsigned long long)ptr;
+}
+
 // The reference's distance from its squared form (idwOperator.cpp:398-402):
 // the square root, or 1e-7 for a coincident sample.
 __device__ __forceinline__ double idw_dist(double s) { return s == 0.0 ? kIdwEps : sqrt(s); }
@@ -420,14 +431,34 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
 
     // the staging registers and their fill level (wave-uniform)
     double sx[kStage], sy[kStage];
-    int sqb[kStage];  // (bucket slot << kBinBits) | bin
+    int sqb[kStage];  // the bucket slot while staging, then (slot << kBinBits) | bin
     int staged = 0;
     // visits the staged candidates bin by bin, nearest bins first; stops at
     // the first bin whose inner edge no lane's threshold reaches.  A
     // candidate's bin b satisfies sqrt(lb2) >= b w (1 - 3e-7) (four float
     // roundings), lb2 <= its squared distance as the test computes it
     // (1 + 2^-51), so with the margins below no skipped candidate passes.
+    // positions and bins of the staged slots: every lane's loads are
+    // requested before the first is used, one wait per flush rather than one
+    // per block
+    auto prepare = [&]() {
+        double2 xy[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; k++)
+            if (64 * k < staged) xy[k] = 64 * k + lane < staged ? p.xy[sqb[k]] : make_double2(NAN, NAN);
+#pragma unroll
+        for (int k = 0; k < kStage; k++) {
+            if (64 * k >= staged) continue;
+            const double ex = fmax(fmax(gx0 - xy[k].x, xy[k].x - gx1), 0.0);
+            const double ey = fmax(fmax(gy0 - xy[k].y, xy[k].y - gy1), 0.0);
+            const float r = __builtin_sqrtf((float)(ex * ex + ey * ey)) * inv_w;
+            sx[k] = xy[k].x;
+            sy[k] = xy[k].y;
+            sqb[k] = (sqb[k] << kBinBits) | (r < (float)(kBins - 1) ? (int)r : kBins - 1);  // NaN: last bin
+        }
+    };
     auto flush = [&]() {
+        prepare();
         for (int bn = 0; bn < kBins; bn++) {
             const double edge = (double)bn * w * (1.0 - 1e-5);
             if (__builtin_amdgcn_ballot_w64(edge * edge * (1.0 - 0x1p-40) <= kthr) == 0) break;
@@ -464,11 +495,13 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
             for (int tCol = blockCol - rad; tCol <= blockCol + rad; tCol += step) {
                 if (tCol < 0 || tCol >= p.blockCols) continue;
                 const int bi = tRow * p.blockCols + tCol;
-                const int s0 = p.start[bi], s1 = p.start[bi + 1];
-                if (act) ncand += (unsigned)p.count[bi];
+                const int s0 = load_uniform(p.start + bi), s1 = load_uniform(p.start + bi + 1);
+                if (act) ncand += (unsigned)load_uniform(p.count + bi);
                 if (s0 == s1) continue;
                 {
-                    const double4 bb = p.bbox[bi];
+                    const double *bbp = reinterpret_cast<const double *>(p.bbox + bi);
+                    const double4 bb = make_double4(load_uniform(bbp), load_uniform(bbp + 1), load_uniform(bbp + 2),
+                                                    load_uniform(bbp + 3));
                     const double dxb = fmax(fmax(bb.x - cellX, cellX - bb.y), 0.0);
                     const double dyb = fmax(fmax(bb.z - cellY, cellY - bb.w), 0.0);
                     const double lb2 = dxb * dxb + dyb * dyb;
@@ -483,16 +516,9 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
                         // registers the block's slots do not reach (wave-uniform)
                         if (64 * k + 64 <= staged || 64 * k >= staged + take) continue;
                         const int i = 64 * k + lane;
-                        if (i >= staged && i < staged + take) {
-                            const int q = q0 + (i - staged);
-                            const double2 xy = p.xy[q];
-                            const double ex = fmax(fmax(gx0 - xy.x, xy.x - gx1), 0.0);
-                            const double ey = fmax(fmax(gy0 - xy.y, xy.y - gy1), 0.0);
-                            const float r = __builtin_sqrtf((float)(ex * ex + ey * ey)) * inv_w;
-                            sx[k] = xy.x;
-                            sy[k] = xy.y;
-                            sqb[k] = (q << kBinBits) | (r < (float)(kBins - 1) ? (int)r : kBins - 1);  // NaN: last bin
-                        }
+                        // only the slot is recorded here; positions and bins
+                        // follow for all staged slots at once (prepare)
+                        if (i >= staged && i < staged + take) sqb[k] = q0 + (i - staged);
                     }
                     staged += take;
                     q0 += take;
@@ -596,11 +622,16 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
             // list's distance slot) and the weighted sum divided once, which
             // moves the result by a few FP64 ulps, far inside the float
             // output's rounding.
+            // unrolled to the list capacity and masked by num, so that the
+            // LDS reads and divisions of all members overlap
             double wsum = 0.0;
-            for (int i = 0; i < num; i++) {
-                const double wi = 1.0 / powi(kd[i * 256], p.power);
-                kd[i * 256] = wi;
-                wsum += wi;
+#pragma unroll
+            for (int i = 0; i < MAXK; i++) {
+                if (i < num) {
+                    const double wi = 1.0 / powi(kd[i * 256], p.power);
+                    kd[i * 256] = wi;
+                    wsum += wi;
+                }
             }
             if (ADJOINT) {
                 const double gc = (double)p.grad_out[(long long)row * p.cols + col];
@@ -609,8 +640,15 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                     for (int i = 0; i < num; i++) adj_add(ks[i * 256], kd[i * 256] * scale);
                 }
             } else {
+                // every member's value is requested before the first is used
+                // (a loop to num would wait for each dependent load in turn)
+                double vals[MAXK];
+#pragma unroll
+                for (int i = 0; i < MAXK; i++) vals[i] = i < num ? p.v[ks[i * 256]] : 0.0;
                 double acc = 0.0;
-                for (int i = 0; i < num; i++) acc += p.v[ks[i * 256]] * kd[i * 256];
+#pragma unroll
+                for (int i = 0; i < MAXK; i++)
+                    if (i < num) acc += vals[i] * kd[i * 256];
                 result = num > 0 ? (float)(acc / wsum) : 0.0f;  // no sample in range: the empty sum, 0
             }
         }
