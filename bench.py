@@ -389,7 +389,13 @@ def secondary_batch(torch, batch, cpu):
     ref_s = [x.clone() for x in b.slopes]
     ref_c = [x.clone() for x in b.codes]
     cpt, rows = b.autotune()
-    gr = event_time(torch, lambda: b.run(batch.GRAPH), 5, warm=1)
+    # graph replays for a fixed wall time first (the host-side tile setup
+    # leaves the GPU idle and its clocks low), then the median of 3 timings
+    w0 = time.perf_counter()
+    while time.perf_counter() - w0 < 0.3:
+        b.run(batch.GRAPH)
+        torch.cuda.synchronize()
+    gr = sorted(event_time(torch, lambda: b.run(batch.GRAPH), 5, warm=1) for _ in range(3))[1]
     # the graph's outputs against the per-tile launches (the autotuned shape
     # may differ from the per-tile one): D8 bit for bit, slope to the bit as
     # well (same arithmetic per cell)
@@ -405,7 +411,7 @@ def secondary_batch(torch, batch, cpu):
     orders = {"estimate_ms": round(gr, 3)}
     for source in ("rows", "measured"):
         b.order(source)
-        ms = event_time(torch, lambda: b.run(batch.GRAPH), 5, warm=1)
+        ms = sorted(event_time(torch, lambda: b.run(batch.GRAPH), 5, warm=1) for _ in range(3))[1]
         orders[source + "_ms"] = round(ms, 3)
         orders[source + "_values_identical"] = all(torch.equal(b.codes[i], ref_c[i]) and
                                                    torch.equal(b.slopes[i], ref_s[i]) for i in range(len(dems)))

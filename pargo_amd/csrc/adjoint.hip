@@ -58,6 +58,10 @@ struct ZRow {
 // the two columns before the wave, lane 63: the two after; other lanes
 // duplicate their own columns).  Loading is split from the conversion so
 // that the next row's loads are in flight while this row is computed.
+// Row vectors need only 4-byte alignment: gfx950 runs a global dwordx4
+// access at any dword address as one instruction.
+typedef float f32x4 __attribute__((ext_vector_type(4), aligned(4)));
+
 struct RawRow {
     float v[ACPT];
     float h0, h1;
@@ -73,7 +77,7 @@ __device__ __forceinline__ void load_raw(const float *__restrict__ base, const A
     if (r >= 0 && r < p.rows) {
         const float *rowp = base + (long long)r * p.pitch;
         if (VEC && full) {
-            const float4 t = *reinterpret_cast<const float4 *>(rowp + c0);
+            const f32x4 t = *reinterpret_cast<const f32x4 *>(rowp + c0);
             o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
         } else {
 #pragma unroll
@@ -242,7 +246,6 @@ __global__ __launch_bounds__(256) void slope_adjoint_kernel(AdjParams p)
         float *dst = p.gdem + (long long)r * p.pitch + c0;
         if (VEC && full) {
             // streamed past the caches: the gradient is not read again here
-            typedef float f32x4 __attribute__((ext_vector_type(4)));
             const f32x4 t = {out[0], out[1], out[2], out[3]};
             __builtin_nontemporal_store(t, reinterpret_cast<f32x4 *>(dst));
         } else {
@@ -297,8 +300,8 @@ extern "C" int pargo_slope_adjoint(pargo_ctx *ctx, const pargo_grid *g, const fl
         p.nd_hi = -INFINITY;
     }
     p.inv8cs = 1.0 / (8.0 * (double)csf);
-    auto al = [](const void *q) { return ((uintptr_t)q % 16) == 0; };
-    const bool vec = pitch % 4 == 0 && al(dem) && al(grad_slope) && al(grad_dem);
+    auto al = [](const void *q) { return ((uintptr_t)q % 4) == 0; };
+    const bool vec = al(dem) && al(grad_slope) && al(grad_dem);
     dim3 block(256);
     dim3 grid((g->cols + 256 * pargo::ACPT - 1) / (256 * pargo::ACPT), (g->rows + p.rows_per_strip - 1) / p.rows_per_strip);
     if (vec) hipLaunchKernelGGL(pargo::slope_adjoint_kernel<true>, grid, block, 0, ctx->stream, p);

@@ -97,12 +97,19 @@ struct FocalParams {
     uint4 *fix_list;      // (tile index, first row, end row, first column)
     unsigned fix_cap;
     unsigned tile_id;     // index of this raster in a batch (0 otherwise)
-    int vec;              // 16-byte aligned rows: vector loads and stores
+    int vec;              // 4-byte aligned rows: vector loads and stores
     int waves_x;          // batch: wave-wide column bands of this tile
 };
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
+// Row vectors at 4-byte alignment and code words at byte alignment: gfx950
+// runs global dwordx4/dwordx2 accesses at any dword address and dword
+// stores at any byte address as single instructions, so a row takes the
+// vector path whatever its pitch (rasters whose width is not a multiple of
+// 4, and batch tiles, included).
+typedef float f32x4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef unsigned u32_b1 __attribute__((aligned(1)));
+typedef unsigned short u16_b1 __attribute__((aligned(1)));
 
 template <int CPT>
 struct RawRow {
@@ -116,8 +123,8 @@ struct RawRow {
 // column instead, which only ever feeds border cells (NoData anyway) or
 // columns that are not stored.  Keeping the loads out of branches lets the
 // compiler wait only for the oldest of the rows in flight.
-// VECLOAD: the wave's 64*CPT columns are inside the raster and the row start
-// is 16-byte aligned (one dwordx4 / dwordx2 per lane).  The halo load is
+// VECLOAD: the wave's 64*CPT columns are inside the raster (one dwordx4 /
+// dwordx2 per lane at any dword address).  The halo load is
 // issued by every lane: lane 0 fetches column c0-1, lane 63 column c0+CPT,
 // the others re-read a column of their own segment (an L1 hit).
 template <int CPT, bool VECLOAD>
@@ -126,10 +133,10 @@ __device__ __forceinline__ void load_raw(const FocalParams &p, int r, int c0, in
     const int rr = r < 0 ? 0 : (r >= p.rows ? p.rows - 1 : r);
     const float *rowp = p.dem + (long long)rr * p.pitch;
     if constexpr (VECLOAD && CPT == 4) {
-        float4 t = *reinterpret_cast<const float4 *>(rowp + (unsigned)c0);
+        const f32x4 t = *reinterpret_cast<const f32x4 *>(rowp + (unsigned)c0);
         o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
     } else if constexpr (VECLOAD && CPT == 2) {
-        float2 t = *reinterpret_cast<const float2 *>(rowp + (unsigned)c0);
+        const f32x2 t = *reinterpret_cast<const f32x2 *>(rowp + (unsigned)c0);
         o.v[0] = t.x; o.v[1] = t.y;
     } else {
 #pragma unroll
@@ -644,10 +651,10 @@ __device__ __forceinline__ void focal_step(const FocalParams &p, int r, int rlas
                 const unsigned packed = __builtin_amdgcn_perm(
                     __builtin_amdgcn_perm(cd[3], cd[2], 0x0c0c0400u), __builtin_amdgcn_perm(cd[1], cd[0], 0x0c0c0400u),
                     0x05040100u);  // byte 0 of each code, three byte permutes
-                __builtin_nontemporal_store(packed, reinterpret_cast<unsigned *>(crow + uc0));
+                __builtin_nontemporal_store(packed, reinterpret_cast<u32_b1 *>(crow + uc0));
             } else if constexpr (CPT == 2) {
                 __builtin_nontemporal_store((unsigned short)(cd[0] | (cd[1] << 8)),
-                                            reinterpret_cast<unsigned short *>(crow + uc0));
+                                            reinterpret_cast<u16_b1 *>(crow + uc0));
             } else {
 #pragma unroll
                 for (int j = 0; j < CPT; j++) crow[uc0 + j] = (uint8_t)cd[j];
@@ -1126,8 +1133,7 @@ static int prepare(pargo_ctx *ctx, const pargo_grid *g, const float *dem, int me
     p.dirtype = code_out ? dirtype : 0;
     p.mask_from_d8 = code_out ? 1 : 0;
     auto aligned = [](const void *ptr, size_t a) { return ((uintptr_t)ptr % a) == 0; };
-    vec = (pitch % 4 == 0) && aligned(dem, 16) && (!slope_out || aligned(slope_out, 16)) &&
-          (!code_out || aligned(code_out, 4));
+    vec = aligned(dem, 4) && (!slope_out || aligned(slope_out, 4));
     p.vec = vec ? 1 : 0;
     return PARGO_OK;
 }
