@@ -153,6 +153,19 @@ __device__ __forceinline__ int mask_index(int other, int self, int idx)
     return (int)((lim < v) ? lim : v);
 }
 
+// 1/x within an ulp for the weights: the hardware reciprocal estimate and
+// two Newton steps, which need no range scaling for x in [2^-1000, 2^1000];
+// other x (and 0, Inf, NaN) take the IEEE division.
+__device__ __forceinline__ double idw_recip(double x)
+{
+    if (!(x >= 0x1p-1000 && x <= 0x1p1000)) return 1.0 / x;
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __device__ __forceinline__ double powi(double d, int p)
 {
     // pow(d, p) for integer p; the result only feeds weights (1e-5 bar).
@@ -586,6 +599,7 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
     unsigned ncand = 0;
     float result = -9999.0f;
     bool pending = inside;
+    int num = 0;  // this lane's list length once its group is searched
     if (inside && p.mask) {
         const int mr = mask_index(p.mask_rows, p.rows, row);
         const int mc = mask_index(p.mask_cols, p.cols, col);
@@ -596,7 +610,9 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
     const double cellY = (double)(p.maxY - (row + 0.5) * (double)p.cs);
     const int blockRow = (int)((p.maxY - cellY) / p.block);
     const int blockCol = (int)((cellX - p.minX) / p.block);
-    // one search per distinct centre block among the pending lanes
+    // one search per distinct centre block among the pending lanes; each
+    // lane's K-list stays in LDS until the weights below
+    const bool searched = pending;
     for (;;) {
         const unsigned long long m = __builtin_amdgcn_ballot_w64(pending);
         if (m == 0) break;
@@ -607,7 +623,8 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         double *kd = kdist + threadIdx.x;
         int *ks = kslot + threadIdx.x;
         bool tie = false;
-        int num = idw_search_binned(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
+        const int n1 = idw_search_binned(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
+        if (mine) num = n1;
         // lanes whose list depends on the visiting order (an exact distance
         // tie at the K-th place) search again in the reference's order
         if (__builtin_amdgcn_ballot_w64(tie) != 0) {
@@ -615,20 +632,27 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
             const int num2 = idw_search_group(p, gRow, gCol, cellX, cellY, tie, kd, ks, again);
             if (tie) num = num2;
         }
-        if (mine) {
+        pending = pending && !mine;
+    }
+    // weights and output once for the whole wave (a wave whose cells span
+    // several centre blocks would otherwise run them once per group)
+    {
+        double *kd = kdist + threadIdx.x;
+        const int *ks = kslot + threadIdx.x;
+        if (searched) {
             // weights 1 / d^p summed in slot order (idwOperator.cpp:472-480).
             // The reference forms each weight twice and divides every term
             // by the sum; here each weight is formed once (kept in the
-            // list's distance slot) and the weighted sum divided once, which
-            // moves the result by a few FP64 ulps, far inside the float
-            // output's rounding.
+            // list's distance slot, 1/d^p within an ulp) and the weighted
+            // sum divided once, which moves the result by a few FP64 ulps,
+            // far inside the float output's rounding.
             // unrolled to the list capacity and masked by num, so that the
             // LDS reads and divisions of all members overlap
             double wsum = 0.0;
 #pragma unroll
             for (int i = 0; i < MAXK; i++) {
                 if (i < num) {
-                    const double wi = 1.0 / powi(kd[i * 256], p.power);
+                    const double wi = idw_recip(powi(kd[i * 256], p.power));
                     kd[i * 256] = wi;
                     wsum += wi;
                 }
@@ -652,7 +676,6 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                 result = num > 0 ? (float)(acc / wsum) : 0.0f;  // no sample in range: the empty sum, 0
             }
         }
-        pending = pending && !mine;
     }
     if (!ADJOINT && inside && p.out) p.out[(long long)row * p.cols + col] = result;
     if (!ADJOINT && inside && p.work_out) p.work_out[(long long)row * p.cols + col] = (double)ncand;
