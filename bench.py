@@ -129,7 +129,7 @@ def copy_probe():
 
 
 CEIL_VARIANTS = {"strip": (0, 9), "strip_halo": (1, 9), "flat": (2, 9), "strip_nt": (3, 9), "flat_nt": (6, 9),
-                 "read": (4, 4), "write": (5, 5)}
+                 "row1_halo": (7, 9), "stack8": (8, 9), "read": (4, 4), "write": (5, 5)}
 
 
 def secondary_ceiling(torch, size):

@@ -19,6 +19,13 @@
 //   STRIP_NT    STRIP with non-temporal stores
 //   READ        STRIP reading only (4 B/cell)
 //   WRITE       STRIP writing only (5 B/cell)
+//   ROW1        STRIP_NT with one-row strips and their two halo rows: every
+//               workgroup reads rows r-1..r+1 and writes row r, so the GPU
+//               sweeps a band of about 64 rows (the fastest form measured;
+//               the halo rows are L2 hits)
+//   STACK8      8 waves stacked on 256 columns walk a 64-row band 8 rows at
+//               a time and pass their rows to each other through LDS, the
+//               other narrow-band form a stencil could take
 //
 // Every access is in bounds for rows x cols with pitch >= cols and
 // cols % 1024 == 0 (checked by the entry point): a wave covers 256 columns,
@@ -34,6 +41,8 @@
 #define PARGO_CEIL_READ 4
 #define PARGO_CEIL_WRITE 5
 #define PARGO_CEIL_FLAT_NT 6
+#define PARGO_CEIL_ROW1 7
+#define PARGO_CEIL_STACK8 8
 
 namespace {
 
@@ -114,6 +123,36 @@ __global__ __launch_bounds__(256) void flat_kernel(const float *__restrict__ src
     }
 }
 
+// STACK8: workgroup = 8 waves on one 256-column segment; each step the
+// waves load rows r0..r0+7, publish them in LDS, read the rows above and
+// below after a barrier (the 3x3 window's rows; their sum is kept alive
+// with a zero weight) and store their own row.
+__global__ __launch_bounds__(512) void stack8_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                                     uint8_t *__restrict__ dst8, int rows, long long pitch,
+                                                     int band, int nseg)
+{
+    __shared__ float4 srow[8][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned seg = blockIdx.x % (unsigned)nseg, b = blockIdx.x / (unsigned)nseg;
+    const unsigned c0 = seg * 256u + (unsigned)lane * 4u;
+    const int rb = (int)b * band, re = min(rb + band, rows);
+    for (int r0 = rb; r0 < re; r0 += 8) {
+        const int r = min(r0 + w, re - 1);
+        float4 v = *reinterpret_cast<const float4 *>(src + (long long)r * pitch + c0);
+        srow[w][lane] = v;
+        __syncthreads();
+        const float4 a = srow[(w + 7) & 7][lane], c = srow[(w + 1) & 7][lane];
+        v.x += a.x * 0.0f + c.x * 0.0f;
+        v.y += a.y * 0.0f + c.y * 0.0f;
+        __syncthreads();
+        if (r0 + w < re) {
+            const f4v t = {v.x, v.y, v.z, v.w};
+            __builtin_nontemporal_store(t, reinterpret_cast<f4v *>(dst + (long long)r * pitch + c0));
+            __builtin_nontemporal_store(pack_codes(v), reinterpret_cast<unsigned *>(dst8 + (long long)r * pitch + c0));
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -150,6 +189,16 @@ int pargo_ceiling_launch(int variant, const float *src, float *dst, uint8_t *dst
         hipLaunchKernelGGL((strip_kernel<false, true, false, false>), grid, block, 0, s, src, dst, dst8, rows, pitch,
                            rows_per_strip, sink);
         break;
+    case PARGO_CEIL_ROW1:
+        hipLaunchKernelGGL((strip_kernel<true, true, true, true>), dim3((unsigned)(cols / 1024), (unsigned)rows), block,
+                           0, s, src, dst, dst8, rows, pitch, 1, sink);
+        break;
+    case PARGO_CEIL_STACK8: {
+        const int nseg = cols / 256, band = 64;
+        hipLaunchKernelGGL(stack8_kernel, dim3((unsigned)(nseg * ((rows + band - 1) / band))), dim3(512), 0, s, src,
+                           dst, dst8, rows, pitch, band, nseg);
+        break;
+    }
     case PARGO_CEIL_FLAT:
     case PARGO_CEIL_FLAT_NT: {
         int dev = 0, cu = 0;
