@@ -246,7 +246,7 @@ int pargo_d8_classify(pargo_ctx *ctx, const pargo_grid *grid, const double *d8, 
  * inflows to NoData; PARGO_SCA_CONVERGE gives the area that repeated sweeps
  * converge to (full D8 flow accumulation).  grid->cellsize is unused.
  * Working memory: the context's scratch grows to 2 bytes per cell (rows and
- * columns rounded up to 8) plus 256 bytes, kept until pargo_destroy. */
+ * columns rounded up to 8) plus 16,640 bytes, kept until pargo_destroy. */
 int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *grid, const uint8_t *flow, const uint32_t *nodata_mask,
                  int mode, double *sca_out);
 

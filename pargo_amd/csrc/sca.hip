@@ -61,6 +61,7 @@
 #include <cmath>
 
 #include "internal.h"
+#include "sca_swar.h"
 
 namespace pargo {
 
@@ -70,6 +71,10 @@ constexpr unsigned kNone = 15;
 constexpr int kTileC = 64, kTileR = 32;  // cells per workgroup: 64 x 32
 constexpr uint64_t kAccMask = (1ull << 40) - 1;
 constexpr uint64_t kOpen = 1ull << 63;   // accumulator tag: not completed
+// The waiting cells are counted by one atomic per init workgroup, spread
+// over 64 counters 256 bytes apart: one counter took 131,072 device-scope
+// atomics per pass at 16384^2, serialised at one address (about 0.3 ms).
+constexpr int kWaitSlots = 64, kWaitStride = 32;  // stride in 8-byte words
 
 __device__ __forceinline__ unsigned flow_of(unsigned b) { return (b <= 8u && b != 4u) ? b : kNone; }
 __device__ __forceinline__ unsigned excl_of(unsigned b)
@@ -87,6 +92,7 @@ struct ScaParams {
     uint16_t *__restrict__ st;          // static words, 8 x 8 blocks
     unsigned long long *__restrict__ counters;  // [0] walk cursor, [1] waiting cells, [2] completions
     unsigned *__restrict__ last_done;   // done flags of the 3 x 3 cells ending at (rows-1, cols-1)
+    unsigned long long *__restrict__ wait_slots;  // waiting cells, kWaitSlots partial counts kWaitStride apart
     long long pitch;
     int rows, cols, mask_words;
     int bx, by;  // 8 x 8 blocks per block row, block rows
@@ -127,6 +133,12 @@ __global__ void sca_reset_kernel(ScaParams p)
     const int i = threadIdx.x;
     if (i < 3) p.counters[i] = 0;
     if (i < 9) p.last_done[i] = 0;
+    if (i < kWaitSlots) p.wait_slots[i * kWaitStride] = 0;
+}
+
+__device__ __forceinline__ void add_waiting(const ScaParams &p, unsigned n)
+{
+    if (n) atomicAdd(&p.wait_slots[(blockIdx.x & (kWaitSlots - 1)) * kWaitStride], (unsigned long long)n);
 }
 
 // Static words and starting values of a 64 x 32 tile.  Classes of the tile
@@ -138,6 +150,92 @@ __global__ void sca_reset_kernel(ScaParams p)
 // neighbouring tiles share their halo lines in one L2.
 __device__ __forceinline__ int win_of_bit(int j) { return j < 4 ? j : j + 1; }
 __device__ __forceinline__ int bit_of_win(int k) { return k < 4 ? k : k - 1; }
+
+typedef unsigned long long u64x2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+
+// A tile at least 2 cells from every border whose class rows can be read
+// as words, four cells per word (sca_swar.h): the halo columns c0 - 8 ..
+// c0 + 71, each word from the aligned pair around it, so bytes c0 - 11 ..
+// c0 + 75 of each row are read.
+__device__ __forceinline__ bool sca_swar_tile(const ScaParams &p, int r0, int c0)
+{
+    return r0 >= 2 && r0 + kSwarTileR + 2 <= p.rows && c0 >= 11 && c0 + 76 <= p.cols;
+}
+
+// The init pass of such a tile, four cells per lane: the same static words
+// and starting values as the per-cell code of sca_init_kernel, with half its
+// VALU instructions and a sixth of its LDS accesses.
+__device__ __forceinline__ void sca_init_words(const ScaParams &p, SwarTile &T, unsigned *waiting, int r0, int c0)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        constexpr int N = kSwarClsRows * kSwarClsWords, PER = (N + 255) / 256;
+        uint32_t lo[PER], hi[PER], mw[PER];
+        unsigned sh[PER];
+        // every load of the thread is issued before the first is used
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + 256 * k;
+            lo[k] = hi[k] = mw[k] = sh[k] = 0;
+            if (i < N) {
+                const int rr = i / kSwarClsWords, w = i - rr * kSwarClsWords;
+                const int r = r0 - 2 + rr, col = c0 - 8 + 4 * w;
+                const uintptr_t a = (uintptr_t)(p.flow + (long long)r * p.pitch + col);
+                const uint32_t *al = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+                sh[k] = (unsigned)(a & 3);
+                lo[k] = al[0];
+                hi[k] = al[1];
+                if (p.mask) mw[k] = p.mask[(long long)r * p.mask_words + (col >> 5)] >> (col & 31);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + 256 * k;
+            if (i < N) {
+                const int rr = i / kSwarClsWords, w = i - rr * kSwarClsWords;
+                const uint32_t v = swar_select(swar_bits_to_mask(mw[k]), swar_splat(PARGO_FLOW_NODATA),
+                                               __builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]));
+                T.cls[rr][w] = v;
+                T.tgt[rr][w] = swar_flow_of(v);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < kSwarInmRows * kSwarInmWords; i += 256) {
+        const int lr = i / kSwarInmWords, q = i - lr * kSwarInmWords;
+        T.inm[lr][q] = swar_inflow(T, lr, q);
+    }
+    __syncthreads();
+    unsigned nwait = 0;
+    // 16 lanes per 8 x 8 block of static words (128 contiguous bytes)
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int v = tid + 256 * k, blk = v >> 4, within = v & 15;
+        const int tr = (blk >> 3) * 8 + (within >> 1), g = (blk & 7) * 2 + (within & 1);
+        const SwarCells o = swar_cells(T, tr, g, p.converge != 0);
+        const int r = r0 + tr, c = c0 + 4 * g;
+        *reinterpret_cast<uint2 *>(p.st + st_index(p, r, c)) = make_uint2(o.s01, o.s23);
+        unsigned long long val[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const unsigned npend = (o.npend >> (8 * j)) & 0xFFu, nsrc = (o.nsrc >> (8 * j)) & 0xFFu;
+            val[j] = ((o.nodata >> (8 * j + 7)) & 1u)  ? bits_of(p.nodata)
+                     : ((o.inert >> (8 * j + 7)) & 1u) ? bits_of(1.0)
+                     : npend                           ? kOpen
+                                                       : bits_of((double)(1u + nsrc));
+        }
+        // streamed past the caches: the walk reads these words later
+        u64x2_a8 *dst = reinterpret_cast<u64x2_a8 *>(p.w + (long long)r * p.pitch + c);
+        const u64x2_a8 v01 = {val[0], val[1]}, v23 = {val[2], val[3]};
+        __builtin_nontemporal_store(v01, dst);
+        __builtin_nontemporal_store(v23, dst + 1);
+        nwait += (unsigned)__builtin_popcount(o.waiting);
+    }
+    nwait = wave_sum(nwait);
+    if (lane == 0) waiting[wave] = nwait;
+    __syncthreads();
+    if (tid == 0) add_waiting(p, waiting[0] + waiting[1] + waiting[2] + waiting[3]);
+}
 
 __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x, int ntiles)
 {
@@ -154,6 +252,12 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
     const int tile = (g & 7) * per + (g >> 3);
     if (tile >= ntiles) return;  // the whole workgroup
     const int r0 = (tile / tiles_x) * kTileR, c0 = (tile % tiles_x) * kTileC;
+    static_assert(kSwarTileR == kTileR && kSwarTileC == kTileC, "one tile shape");
+    if (sca_swar_tile(p, r0, c0)) {
+        __shared__ SwarTile T;
+        sca_init_words(p, T, waiting, r0, c0);
+        return;
+    }
     {
         // every load of the thread is issued before the first store to LDS,
         // so that their latencies overlap
@@ -261,7 +365,7 @@ __global__ __launch_bounds__(256) void sca_init_kernel(ScaParams p, int tiles_x,
     __syncthreads();
     if (tid == 0) {
         const unsigned n = waiting[0] + waiting[1] + waiting[2] + waiting[3];
-        if (n) atomicAdd(&p.counters[1], (unsigned long long)n);
+        add_waiting(p, n);
     }
     // the tile's 4 x 8 blocks, one wave-wide 128-byte line per block
     for (int i = tid; i < kTileR * kTileC; i += 256) {
@@ -391,7 +495,9 @@ __global__ __launch_bounds__(256) void sca_walk_kernel(ScaParams p)
 // the done arrivals they received, as the converging semantics defines it.
 __global__ __launch_bounds__(256) void sca_open_kernel(ScaParams p)
 {
-    if (p.counters[1] == p.counters[2]) return;
+    unsigned long long waiting = 0;
+    for (int i = 0; i < kWaitSlots; i++) waiting += p.wait_slots[i * kWaitStride];
+    if (waiting == p.counters[2]) return;
     const long long n = (long long)p.rows * p.cols;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
         const int r = (int)(i / p.cols), c = (int)(i - (long long)r * p.cols);
@@ -519,10 +625,11 @@ extern "C" int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *g, const uint8_t *
     if (ntiles > (1ll << 24))
         return fail(ctx, PARGO_E_INVALID, "grid of %d x %d cells needs %lld tiles, more than 2^24", g->rows, g->cols,
                     ntiles);
-    // scratch: counters and last-cell flags, then the static words in 8 x 8
-    // blocks (2 bytes per cell, rows and columns rounded up to 8)
+    // scratch: counters and last-cell flags, the waiting-cell partial
+    // counts, then the static words in 8 x 8 blocks (2 bytes per cell, rows
+    // and columns rounded up to 8)
     const int bx = (g->cols + 7) / 8, by = (g->rows + 7) / 8;
-    const size_t head = 256;
+    const size_t head = 256 + (size_t)pargo::kWaitSlots * pargo::kWaitStride * sizeof(unsigned long long);
     rc = pargo::ensure_scratch(ctx, head + (size_t)bx * by * 64 * sizeof(uint16_t));
     if (rc) return rc;
     char *scr = static_cast<char *>(ctx->scratch);
@@ -533,6 +640,7 @@ extern "C" int pargo_sca_d8(pargo_ctx *ctx, const pargo_grid *g, const uint8_t *
     p.st = reinterpret_cast<uint16_t *>(scr + head);
     p.counters = reinterpret_cast<unsigned long long *>(scr);
     p.last_done = reinterpret_cast<unsigned *>(scr + 64);
+    p.wait_slots = reinterpret_cast<unsigned long long *>(scr + 256);
     p.pitch = pitch;
     p.rows = g->rows;
     p.cols = g->cols;

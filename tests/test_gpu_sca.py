@@ -53,6 +53,22 @@ def test_random_codes_match_oracle(seed, mode):
     assert np.array_equal(got, ref), np.argwhere(got != ref)[:5].tolist()
 
 
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("mode", ["reference", "converge"])
+def test_random_codes_interior_tiles_match_oracle(seed, mode):
+    """Rasters large enough for the init pass's word-wise interior tiles
+    (64 x 32 cells, 2+ cells from the border, class rows read as aligned
+    words), every class kind, row pitches at every byte offset."""
+    rng = np.random.default_rng(2000 + seed)
+    rows, cols = 200 + int(rng.integers(0, 40)), 400 + int(rng.integers(0, 80))
+    choice = [-1.0, 0, 1, 2, 3, 4, 5, 6, 7, 8, ND, np.nan, 2.9999999999999996, 5.5]
+    p = np.array([4, 8, 8, 8, 2, 8, 8, 8, 8, 1, 1, 1, 1, 1], dtype=np.float64)
+    d8 = rng.choice(choice, size=(rows, cols), p=p / p.sum())
+    ref, _ = pyoracle.sca_d8(d8, ND, mode == "converge")
+    got = gpu_from_double(d8, ND, mode, pitch_pad=seed)
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5].tolist()
+
+
 @pytest.mark.parametrize("mode", ["reference", "converge"])
 def test_dem_d8_chain_matches_oracle(mode):
     import torch
