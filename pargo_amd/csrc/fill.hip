@@ -104,15 +104,19 @@ __device__ __forceinline__ void schedule(const FillParams &p, int t, int pass)
 // W before the first sweep (pitRemoveOperator.cpp:39-45 plus copyLayerInfo),
 // and pass 0's worklist: every tile holding a NoData or border cell in its
 // one-cell halo.
-__global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
+// kInitRows rows of 256 columns per workgroup: one workgroup per row
+// segment spent most of the pass starting workgroups (a million of them at
+// 16384^2).
+constexpr int kInitRows = 16;
+
+__device__ __forceinline__ void fill_init_cell(const FillParams &p, int r, int c)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x, r = blockIdx.y;
-    if (c >= p.cols) return;
     const long long i = (long long)r * p.pitch + c;
     const bool interior = r >= 1 && r <= p.rows - 2 && c >= 1 && c <= p.cols - 2;
     const bool valid = interior && !is_nd(p, p.dem[i]);
     const double w0 = valid ? kStart : p.nodata;
-    p.w[i] = w0;
+    // streamed past the caches: the passes read W back much later
+    __builtin_nontemporal_store(w0, &p.w[i]);
     if (c % kTW == 0 || c % kTW == kTW - 1)
         p.edge[edge_index(p, r / kTH, c / kTW, c % kTW != 0, r % kTH)] = w0;
     // The tiles whose cells or halo hold an excluded cell of this wave.  A
@@ -134,6 +138,14 @@ __global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
             // go from -1 to 0 here, so a stale -1 costs an atomic, nothing more
             if (p.stamp[t] != 0) schedule(p, t, 0);
         }
+}
+
+__global__ __launch_bounds__(256) void fill_init_kernel(FillParams p)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x, rb = blockIdx.y * kInitRows;
+    if (c >= p.cols) return;  // whole waves: 256-column segments hold 4 of them
+    const int re = min(rb + kInitRows, p.rows);
+    for (int r = rb; r < re; r++) fill_init_cell(p, r, c);
 }
 
 // Stamps to -1 and both worklists empty (before the first pass), or the
@@ -345,17 +357,19 @@ __global__ __launch_bounds__(256) void fill_range_kernel(FillParams p, unsigned 
 {
     __shared__ unsigned smin[4], sbad[4];
     unsigned u = 0xffffffffu, bad = 0;
-    const long long n = (long long)p.rows * p.cols;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-        const int r = (int)(i / p.cols), c = (int)(i - (long long)r * p.cols);
-        if (r < 1 || r > p.rows - 2 || c < 1 || c > p.cols - 2) continue;
-        const float f = p.dem[(long long)r * p.pitch + c];
-        if (f != f) {
-            bad++;
-        } else if (!is_nd(p, f)) {
-            unsigned k = __float_as_uint(f);  // order-preserving map of floats to unsigned
-            k = (k & 0x80000000u) ? ~k : (k | 0x80000000u);
-            u = k < u ? k : u;
+    // interior rows by workgroup, their interior columns by thread (a flat
+    // index would need a 64-bit division per cell)
+    for (int r = 1 + (int)blockIdx.x; r <= p.rows - 2; r += (int)gridDim.x) {
+        const float *row = p.dem + (long long)r * p.pitch;
+        for (int c = 1 + (int)threadIdx.x; c <= p.cols - 2; c += 256) {
+            const float f = row[c];
+            if (f != f) {
+                bad++;
+            } else if (!is_nd(p, f)) {
+                unsigned k = __float_as_uint(f);  // order-preserving map of floats to unsigned
+                k = (k & 0x80000000u) ? ~k : (k | 0x80000000u);
+                u = k < u ? k : u;
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -434,7 +448,7 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
     // monotonicity check: every valid W lies in [min(dem, 10000), 10000]
     unsigned host2[2] = {0xffffffffu, 0u};
     PARGO_HIP(ctx, hipMemcpyAsync(scr, host2, sizeof(host2), hipMemcpyHostToDevice, ctx->stream));
-    dim3 rowsgrid((g->cols + 255) / 256, g->rows);
+    dim3 initgrid((g->cols + 255) / 256, (g->rows + pargo::kInitRows - 1) / pargo::kInitRows);
     const int cus = ctx->num_cu > 0 ? ctx->num_cu : 256;
     hipLaunchKernelGGL(pargo::fill_range_kernel, dim3(cus * 8), dim3(256), 0, ctx->stream, p, minbits, nans);
     PARGO_HIP(ctx, hipMemcpyAsync(host2, scr, sizeof(host2), hipMemcpyDeviceToHost, ctx->stream));
@@ -458,7 +472,7 @@ extern "C" int pargo_pit_remove(pargo_ctx *ctx, const pargo_grid *g, const float
     // stamps at -1 and empty worklists, then W and pass 0's worklist
     hipLaunchKernelGGL(pargo::fill_stamp_init_kernel, dim3((std::max(ntiles, kBatch) + 255) / 256), dim3(256), 0,
                        ctx->stream, p.stamp, ntiles, changed, kBatch, p.count);
-    hipLaunchKernelGGL(pargo::fill_init_kernel, rowsgrid, dim3(256), 0, ctx->stream, p);
+    hipLaunchKernelGGL(pargo::fill_init_kernel, initgrid, dim3(256), 0, ctx->stream, p);
     // a fixed grid walks each pass's worklist: as many workgroups as are
     // resident at once
     int per_cu = 0;
