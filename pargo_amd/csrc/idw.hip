@@ -539,12 +539,20 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
                 }
             }
         }
-        if (staged) flush();
+        // Ring 0 never ends a search (its range is negative), so its
+        // candidates may wait for ring 1's flush when no lane would leave
+        // before ring 1 (the buffer rule at range minDistToBound) and a ring
+        // 1 exists: the visited set is the same, one position-load wait
+        // fewer.
+        const bool defer = rad == 0 && p.maxRing >= 1 &&
+                           __builtin_amdgcn_ballot_w64(act && p.buffer > 0 && minDistToBound >= p.buffer) == 0;
+        if (staged && !defer) flush();
         if (act && tail >= K - 1 && searchRange >= maxDist) {
             act = false;
             kthr = -INFINITY;
         }
     }
+    if (staged) flush();  // never reached with candidates left: ring 1 flushes
     tie = mine && tail >= K - 1 && runner == maxDist;
     return (tail + 1 < K) ? tail + 1 : K;
 }
