@@ -378,6 +378,7 @@ constexpr int kStage = 4;     // staged candidates per lane: 256 per wave
 constexpr int kBins = 48;     // bins of block / 16 each, the last one open
 constexpr int kBinBits = 6;   // a staged entry packs (slot << 6) | bin
 
+template <int MAXK>
 __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRow, int blockCol, double cellX,
                                                  double cellY, bool mine, double *kd, int *ks, unsigned &ncand,
                                                  bool &tie)
@@ -426,15 +427,22 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
                 runner = fmin(runner, maxDist);  // the evicted distance
                 kd[maxIdx * 256] = tmpDist;
                 ks[maxIdx * 256] = q;
-                maxDist = kd[0];
-                maxIdx = 0;
-                for (int i = 1; i < K; i++) {
-                    const double di = kd[i * 256];
-                    if (di > maxDist) {
-                        maxDist = di;
-                        maxIdx = i;
-                    }
-                }
+                // the new largest distance and its first slot, as the
+                // reference's strict-> rescan finds them; every entry is
+                // read before the first compare (one LDS wait, not K).
+                // Slots K .. MAXK - 1 hold 0, below every distance.
+                double di[MAXK];
+#pragma unroll
+                for (int i = 0; i < MAXK; i++) di[i] = kd[i * 256];
+                double mx = di[0];
+#pragma unroll
+                for (int i = 1; i < MAXK; i++) mx = fmax(mx, di[i]);
+                int mi = 0;
+#pragma unroll
+                for (int i = MAXK - 1; i >= 0; i--)
+                    if (di[i] == mx) mi = i;
+                maxDist = mx;
+                maxIdx = mi;
                 kthr = maxDist * maxDist * kPrefilterSlack;
             } else {
                 runner = fmin(runner, tmpDist);
@@ -595,6 +603,9 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         }
         atomicAdd(&p.grad_acc[q], v);
     };
+    // list slots beyond K: 0, so that the unrolled rescan of a full list
+    // never picks one (every distance is at least 1e-7)
+    for (int i = p.k; i < MAXK; i++) kdist[i * 256 + threadIdx.x] = 0.0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // one workgroup per 16 x 16 tile, launched in row-major tile order or
     // in the order of a load model (heaviest first)
@@ -631,7 +642,7 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         double *kd = kdist + threadIdx.x;
         int *ks = kslot + threadIdx.x;
         bool tie = false;
-        const int n1 = idw_search_binned(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
+        const int n1 = idw_search_binned<MAXK>(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
         if (mine) num = n1;
         // lanes whose list depends on the visiting order (an exact distance
         // tie at the K-th place) search again in the reference's order
