@@ -516,18 +516,19 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
             for (int tCol = blockCol - rad; tCol <= blockCol + rad; tCol += step) {
                 if (tCol < 0 || tCol >= p.blockCols) continue;
                 const int bi = tRow * p.blockCols + tCol;
+                // the block's start, end, count and bounding box are all
+                // requested before any is used (one wait per block)
                 const int s0 = load_uniform(p.start + bi), s1 = load_uniform(p.start + bi + 1);
-                if (act) ncand += (unsigned)load_uniform(p.count + bi);
-                if (s0 == s1) continue;
-                {
-                    const double *bbp = reinterpret_cast<const double *>(p.bbox + bi);
-                    const double4 bb = make_double4(load_uniform(bbp), load_uniform(bbp + 1), load_uniform(bbp + 2),
-                                                    load_uniform(bbp + 3));
-                    const double dxb = fmax(fmax(bb.x - cellX, cellX - bb.y), 0.0);
-                    const double dyb = fmax(fmax(bb.z - cellY, cellY - bb.w), 0.0);
-                    const double lb2 = dxb * dxb + dyb * dyb;
-                    if (__builtin_amdgcn_ballot_w64(lb2 <= kthr) == 0) continue;
-                }
+                const int cnt = load_uniform(p.count + bi);
+                const double *bbp = reinterpret_cast<const double *>(p.bbox + bi);
+                const double4 bb = make_double4(load_uniform(bbp), load_uniform(bbp + 1), load_uniform(bbp + 2),
+                                                load_uniform(bbp + 3));
+                ncand += act ? (unsigned)cnt : 0u;
+                const double dxb = fmax(fmax(bb.x - cellX, cellX - bb.y), 0.0);
+                const double dyb = fmax(fmax(bb.z - cellY, cellY - bb.w), 0.0);
+                const double lb2 = dxb * dxb + dyb * dyb;
+                const unsigned long long reach = __builtin_amdgcn_ballot_w64(lb2 <= kthr);
+                if (s0 == s1 || reach == 0) continue;
                 // stage the block's slots (padding slots hold NaN positions,
                 // which no test passes)
                 for (int q0 = s0; q0 < s1;) {
