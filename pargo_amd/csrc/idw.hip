@@ -681,7 +681,16 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                 const double gc = (double)p.grad_out[(long long)row * p.cols + col];
                 if (gc != 0.0 && num > 0) {
                     const double scale = gc / wsum;
-                    for (int i = 0; i < num; i++) adj_add(ks[i * 256], kd[i * 256] * scale);
+                    // Neighbouring cells share most members, mostly in the
+                    // same list slots, and LDS atomics of one instruction
+                    // on one address are serialized: each lane starts at
+                    // slot lane mod num, so one instruction's lanes spread
+                    // over the slots.
+                    int i = lane % num;
+                    for (int t = 0; t < num; t++) {
+                        adj_add(ks[i * 256], kd[i * 256] * scale);
+                        i = i + 1 == num ? 0 : i + 1;
+                    }
                 }
             } else {
                 // every member's value is requested before the first is used
