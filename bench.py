@@ -23,7 +23,8 @@ Unless --no-secondary is given it also measures, under "secondary":
 Before the --warmup steps, the headline step runs for --settle-ms of wall
 time so that the GPU clocks have left their idle state (the DEM is
 generated on the host first, and the GPU idles meanwhile); the JSON line
-states this under "clock_settle".
+states this under "clock_settle".  The adjoint, IDW, scad8 and pitremove legs
+likewise run their kernels for 100-200 ms before timing them.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
@@ -87,6 +88,16 @@ def host_cpu():
         pass
     info["threads_used"] = threads
     return info
+
+
+def settle(torch, fn, ms=200.0):
+    """Runs fn for about ms of wall time so that the GPU clocks leave the
+    state an idle spell (host-side data generation) left them in."""
+    w0 = time.perf_counter()
+    while (time.perf_counter() - w0) * 1000.0 < ms:
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
 
 
 def event_time(torch, fn, n, warm=2):
@@ -180,11 +191,20 @@ def secondary_adjoint(torch, _lib, focal, synth):
     ctx.bind_stream()
     L = _lib.lib()
     grid = focal.grid_of(dem, CELL, NODATA)
-    fwd = event_time(torch, lambda: ctx.check(L.pargo_slope(ctx.h, ctypes.byref(grid), _lib._ptr(dem), 0, 0,
-                                                            _lib._ptr(s), None)), 20)
-    adj = event_time(torch, lambda: ctx.check(L.pargo_slope_adjoint(ctx.h, ctypes.byref(grid), _lib._ptr(dem),
-                                                                    _lib._ptr(g), _lib._ptr(gd))), 20)
+    def fwd_step():
+        ctx.check(L.pargo_slope(ctx.h, ctypes.byref(grid), _lib._ptr(dem), 0, 0, _lib._ptr(s), None))
+
+    def adj_step():
+        ctx.check(L.pargo_slope_adjoint(ctx.h, ctypes.byref(grid), _lib._ptr(dem), _lib._ptr(g), _lib._ptr(gd)))
+
+    # both at settled clocks (the DEM's generation leaves the GPU idle); the
+    # median of three timings of 20 launches each
+    settle(torch, fwd_step)
+    fwd = sorted(event_time(torch, fwd_step, 20) for _ in range(3))[1]
+    settle(torch, adj_step)
+    adj = sorted(event_time(torch, adj_step, 20) for _ in range(3))[1]
     out = {"config": "Horn slope and its adjoint on an 8192^2 float32 DEM (1% NoData)",
+           "timing": "each after 200 ms of its own launches, median of 3 x 20 launches",
            "fwd_ms": round(fwd, 5), "adjoint_ms": round(adj, 5), "adjoint_fwd_ratio": round(adj / fwd, 3),
            "fwd_GBs": round(8 * N * N / fwd / 1e6, 1), "adjoint_GBs": round(12 * N * N / adj / 1e6, 1),
            "adjoint_bytes_per_cell": 12}
@@ -246,7 +266,10 @@ def secondary_idw(torch, synth, IDWOperator, cpu):
     out, cand = op.run(count_candidates=True)
     torch.cuda.synchronize()
     c = int(cand.item())
-    ms = event_time(torch, lambda: op.run(out=out), 5, warm=1)
+    # timings at settled clocks: 200 ms of launches first, then the median
+    # of three timings of 5 launches
+    settle(torch, lambda: op.run(out=out))
+    ms = sorted(event_time(torch, lambda: op.run(out=out), 5, warm=1) for _ in range(3))[1]
     cells = op.rows * op.cols
     flop = 5.0 * c + 49.0 * cells
     res = {"config": "IDW of 20000 samples onto 4096^2 cells, K=12, p=2, searchRange 5000, blockSize 5000",
@@ -263,7 +286,8 @@ def secondary_idw(torch, synth, IDWOperator, cpu):
         o2 = op.run()
         torch.cuda.synchronize()
         same = bool(torch.equal(o2, ref_out))
-        orders[source] = {"ms": round(event_time(torch, lambda: op.run(out=o2), 5, warm=1), 4), "values_identical": same}
+        oms = sorted(event_time(torch, lambda: op.run(out=o2), 5, warm=1) for _ in range(3))[1]
+        orders[source] = {"ms": round(oms, 4), "values_identical": same}
     op.order_tiles("rows")
     res["tile_order"] = {"rows_ms": res["ms"], **orders}
     # the IDW adjoint (dL/d sample values) for an L2 loss against a seeded
@@ -271,7 +295,8 @@ def secondary_idw(torch, synth, IDWOperator, cpu):
     gen = torch.Generator(device="cuda").manual_seed(synth.BASE_SEED + 3)
     target = out + torch.randn(out.shape, device="cuda", generator=gen)
     grad = (2.0 * (out - target)).contiguous()
-    ams = event_time(torch, lambda: op.adjoint(grad), 3, warm=1)
+    settle(torch, lambda: op.adjoint(grad), 100.0)
+    ams = sorted(event_time(torch, lambda: op.adjoint(grad), 3, warm=1) for _ in range(3))[1]
     res["adjoint_ms"] = round(ams, 4)
     res["adjoint_fwd_ratio"] = round(ams / ms, 3)
     if cpu is not None:
@@ -312,7 +337,9 @@ def secondary_sca(torch, synth, focal, hydro, z, cpu):
                      "float64 areas out)"}
     cells = rows * cols
     for mode in ("reference", "converge"):
-        ms = event_time(torch, lambda: hydro.sca_d8(code, NODATA, mask, mode), 5, warm=2)
+        settle(torch, lambda: hydro.sca_d8(code, NODATA, mask, mode))
+        ms = sorted(event_time(torch, lambda: hydro.sca_d8(code, NODATA, mask, mode), 5, warm=1)
+                    for _ in range(3))[1]
         res[mode + "_ms"] = round(ms, 3)
         res[mode + "_Gcells_s"] = round(cells / ms / 1e6, 2)
     # 1 B code + 1/8 B mask in, 8 B area out per cell
@@ -343,8 +370,7 @@ def secondary_fill(torch, synth, hydro, z, cpu):
     """Depression filling (pitremove) of the headline DEM."""
     rows, cols = z.shape
     dem = torch.from_numpy(z).cuda()
-    hydro.pit_remove(dem, NODATA)
-    torch.cuda.synchronize()
+    settle(torch, lambda: hydro.pit_remove(dem, NODATA))
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
