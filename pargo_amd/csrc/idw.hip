@@ -207,13 +207,14 @@ __device__ __forceinline__ void load8_uniform(const double2 *src, double2 c[8])
 
 // The K-nearest search of the lanes whose cell lies in sample block
 // (blockRow, blockCol) (wave-uniform) and that have `mine` set.  Each lane's
-// list of (distance, bucket slot) pairs lives in LDS at kd[i * 256] /
-// ks[i * 256], i < K (the caller offsets the pointers by the lane's thread
+// list of (distance, bucket slot) pairs lives in LDS at kd[i * NT] /
+// ks[i * NT], i < K (the caller offsets the pointers by the lane's thread
 // index): the list is written on an insertion and read on a replacement's
 // rescan and at the end, while the test of a candidate needs only the
 // running threshold in registers.  Fills the list like nbrSamples and
 // returns min(found, K).  Counts the samples of the visited blocks in
 // *ncand.
+template <int NT>
 __device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow, int blockCol, double cellX,
                                                 double cellY, bool mine, double *kd, int *ks, unsigned &ncand)
 {
@@ -247,8 +248,8 @@ __device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow
             const double tmpDist = idw_dist(s);
             if (tail < K - 1) {
                 tail++;
-                kd[tail * 256] = tmpDist;
-                ks[tail * 256] = q;
+                kd[tail * NT] = tmpDist;
+                ks[tail * NT] = q;
                 if (tmpDist > maxDist) {
                     maxDist = tmpDist;
                     maxIdx = tail;
@@ -257,12 +258,12 @@ __device__ __forceinline__ int idw_search_group(const IdwParams &p, int blockRow
             } else if (tmpDist < maxDist) {
                 // (the reference also counts tailIdx up here; past K-1 only
                 // "tail >= K-1" is ever used)
-                kd[maxIdx * 256] = tmpDist;
-                ks[maxIdx * 256] = q;
+                kd[maxIdx * NT] = tmpDist;
+                ks[maxIdx * NT] = q;
                 maxDist = kd[0];
                 maxIdx = 0;
                 for (int i = 1; i < K; i++) {
-                    const double di = kd[i * 256];
+                    const double di = kd[i * NT];
                     if (di > maxDist) {
                         maxDist = di;
                         maxIdx = i;
@@ -378,7 +379,7 @@ constexpr int kStage = 4;     // staged candidates per lane: 256 per wave
 constexpr int kBins = 48;     // bins of block / 16 each, the last one open
 constexpr int kBinBits = 6;   // a staged entry packs (slot << 6) | bin
 
-template <int MAXK>
+template <int MAXK, int NT>
 __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRow, int blockCol, double cellX,
                                                  double cellY, bool mine, double *kd, int *ks, unsigned &ncand,
                                                  bool &tie)
@@ -416,8 +417,8 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
             const double tmpDist = idw_dist(s);
             if (tail < K - 1) {
                 tail++;
-                kd[tail * 256] = tmpDist;
-                ks[tail * 256] = q;
+                kd[tail * NT] = tmpDist;
+                ks[tail * NT] = q;
                 if (tmpDist > maxDist) {
                     maxDist = tmpDist;
                     maxIdx = tail;
@@ -425,15 +426,15 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
                 if (tail == K - 1) kthr = maxDist * maxDist * kPrefilterSlack;
             } else if (tmpDist < maxDist) {
                 runner = fmin(runner, maxDist);  // the evicted distance
-                kd[maxIdx * 256] = tmpDist;
-                ks[maxIdx * 256] = q;
+                kd[maxIdx * NT] = tmpDist;
+                ks[maxIdx * NT] = q;
                 // the new largest distance and its first slot, as the
                 // reference's strict-> rescan finds them; every entry is
                 // read before the first compare (one LDS wait, not K).
                 // Slots K .. MAXK - 1 hold 0, below every distance.
                 double di[MAXK];
 #pragma unroll
-                for (int i = 0; i < MAXK; i++) di[i] = kd[i * 256];
+                for (int i = 0; i < MAXK; i++) di[i] = kd[i * NT];
                 double mx = di[0];
 #pragma unroll
                 for (int i = 1; i < MAXK; i++) mx = fmax(mx, di[i]);
@@ -579,13 +580,22 @@ constexpr int kAdjTableBits = 8;
 constexpr int kAdjTable = 1 << kAdjTableBits;
 constexpr int kAdjProbe = 16;
 
+// The forward runs one wave per workgroup (an 8 x 8 quarter of a 16 x 16
+// tile, four consecutive workgroups per tile), so that a wave's registers
+// and LDS are freed when it ends rather than when the slowest wave of its
+// tile does; the adjoint keeps four waves per workgroup, which share the
+// hash table.  NT: lanes per workgroup, the stride of the K-lists in LDS.
+template <bool ADJOINT>
+constexpr int idw_lanes() { return ADJOINT ? 256 : 64; }
+
 template <int MAXK, bool ADJOINT>
-__global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
+__global__ __launch_bounds__(idw_lanes<ADJOINT>()) void idw_kernel(IdwParams p)
 {
+    constexpr int NT = idw_lanes<ADJOINT>();
     __shared__ int tkey[ADJOINT ? kAdjTable : 1];
     __shared__ double tval[ADJOINT ? kAdjTable : 1];
-    __shared__ double kdist[MAXK * 256];  // [slot i][thread]
-    __shared__ int kslot[MAXK * 256];
+    __shared__ double kdist[MAXK * NT];  // [slot i][thread]
+    __shared__ int kslot[MAXK * NT];
     if constexpr (ADJOINT) {
         for (int i = threadIdx.x; i < kAdjTable; i += blockDim.x) {
             tkey[i] = -1;
@@ -606,12 +616,14 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
     };
     // list slots beyond K: 0, so that the unrolled rescan of a full list
     // never picks one (every distance is at least 1e-7)
-    for (int i = p.k; i < MAXK; i++) kdist[i * 256 + threadIdx.x] = 0.0;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // one workgroup per 16 x 16 tile, launched in row-major tile order or
-    // in the order of a load model (heaviest first)
+    for (int i = p.k; i < MAXK; i++) kdist[i * NT + threadIdx.x] = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int wave = NT == 256 ? (int)(threadIdx.x >> 6) : (int)(blockIdx.x & 3);
+    // 16 x 16 tiles, launched in row-major tile order or in the order of a
+    // load model (heaviest first)
+    const unsigned tslot = NT == 256 ? blockIdx.x : blockIdx.x >> 2;
     const int tiles_x = (p.cols + 15) >> 4;
-    const int tile = p.order ? p.order[blockIdx.x] : (int)blockIdx.x;
+    const int tile = p.order ? p.order[tslot] : (int)tslot;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
     const int row = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int col = tx * 16 + (wave & 1) * 8 + (lane & 7);
@@ -643,13 +655,13 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
         double *kd = kdist + threadIdx.x;
         int *ks = kslot + threadIdx.x;
         bool tie = false;
-        const int n1 = idw_search_binned<MAXK>(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
+        const int n1 = idw_search_binned<MAXK, NT>(p, gRow, gCol, cellX, cellY, mine, kd, ks, ncand, tie);
         if (mine) num = n1;
         // lanes whose list depends on the visiting order (an exact distance
         // tie at the K-th place) search again in the reference's order
         if (__builtin_amdgcn_ballot_w64(tie) != 0) {
             unsigned again = 0;
-            const int num2 = idw_search_group(p, gRow, gCol, cellX, cellY, tie, kd, ks, again);
+            const int num2 = idw_search_group<NT>(p, gRow, gCol, cellX, cellY, tie, kd, ks, again);
             if (tie) num = num2;
         }
         pending = pending && !mine;
@@ -672,8 +684,8 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
 #pragma unroll
             for (int i = 0; i < MAXK; i++) {
                 if (i < num) {
-                    const double wi = idw_recip(powi(kd[i * 256], p.power));
-                    kd[i * 256] = wi;
+                    const double wi = idw_recip(powi(kd[i * NT], p.power));
+                    kd[i * NT] = wi;
                     wsum += wi;
                 }
             }
@@ -688,7 +700,7 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                     // over the slots.
                     int i = lane % num;
                     for (int t = 0; t < num; t++) {
-                        adj_add(ks[i * 256], kd[i * 256] * scale);
+                        adj_add(ks[i * NT], kd[i * NT] * scale);
                         i = i + 1 == num ? 0 : i + 1;
                     }
                 }
@@ -697,11 +709,11 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
                 // (a loop to num would wait for each dependent load in turn)
                 double vals[MAXK];
 #pragma unroll
-                for (int i = 0; i < MAXK; i++) vals[i] = i < num ? p.v[ks[i * 256]] : 0.0;
+                for (int i = 0; i < MAXK; i++) vals[i] = i < num ? p.v[ks[i * NT]] : 0.0;
                 double acc = 0.0;
 #pragma unroll
                 for (int i = 0; i < MAXK; i++)
-                    if (i < num) acc += vals[i] * kd[i * 256];
+                    if (i < num) acc += vals[i] * kd[i * NT];
                 result = num > 0 ? (float)(acc / wsum) : 0.0f;  // no sample in range: the empty sum, 0
             }
         }
@@ -726,8 +738,9 @@ __global__ __launch_bounds__(256) void idw_kernel(IdwParams p)
 template <int MAXK, bool ADJOINT>
 static void launch_idw_k(const IdwParams &p, hipStream_t s)
 {
-    dim3 block(256);
-    dim3 grid((unsigned)(((p.cols + 15) / 16) * ((p.rows + 15) / 16)));
+    constexpr int nt = idw_lanes<ADJOINT>();
+    dim3 block(nt);
+    dim3 grid((unsigned)(((p.cols + 15) / 16) * ((p.rows + 15) / 16)) * (256 / nt));
     hipLaunchKernelGGL((idw_kernel<MAXK, ADJOINT>), grid, block, 0, s, p);
 }
 
