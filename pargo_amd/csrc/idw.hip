@@ -573,25 +573,23 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
 // an LDS hash table (open addressing on the bucket slot) and flushed with
 // one global FP64 atomic per distinct sample; a crowded table falls back to
 // global atomics.
-// 256 entries (3 KiB) keep the table and the K-lists of a workgroup within
-// 40 KiB of LDS, four workgroups per CU; a 16x16 tile selects a few dozen
-// distinct samples.
-constexpr int kAdjTableBits = 8;
+// 64 entries (768 bytes) keep the table and the K-lists of a one-wave
+// workgroup within 10 KiB of LDS (K = 12), four waves per SIMD; an 8 x 8
+// quarter tile selects a few dozen distinct samples at most.
+constexpr int kAdjTableBits = 6;
 constexpr int kAdjTable = 1 << kAdjTableBits;
 constexpr int kAdjProbe = 16;
 
-// The forward runs one wave per workgroup (an 8 x 8 quarter of a 16 x 16
-// tile, four consecutive workgroups per tile), so that a wave's registers
-// and LDS are freed when it ends rather than when the slowest wave of its
-// tile does; the adjoint keeps four waves per workgroup, which share the
-// hash table.  NT: lanes per workgroup, the stride of the K-lists in LDS.
-template <bool ADJOINT>
-constexpr int idw_lanes() { return ADJOINT ? 256 : 64; }
+// One wave per workgroup (an 8 x 8 quarter of a 16 x 16 tile, four
+// consecutive workgroups per tile), so that a wave's registers and LDS are
+// freed when it ends rather than when the slowest wave of its tile does.
+// kIdwLanes: lanes per workgroup, the stride of the K-lists in LDS.
+constexpr int kIdwLanes = 64;
 
 template <int MAXK, bool ADJOINT>
-__global__ __launch_bounds__(idw_lanes<ADJOINT>()) void idw_kernel(IdwParams p)
+__global__ __launch_bounds__(kIdwLanes) void idw_kernel(IdwParams p)
 {
-    constexpr int NT = idw_lanes<ADJOINT>();
+    constexpr int NT = kIdwLanes;
     __shared__ int tkey[ADJOINT ? kAdjTable : 1];
     __shared__ double tval[ADJOINT ? kAdjTable : 1];
     __shared__ double kdist[MAXK * NT];  // [slot i][thread]
@@ -617,11 +615,11 @@ __global__ __launch_bounds__(idw_lanes<ADJOINT>()) void idw_kernel(IdwParams p)
     // list slots beyond K: 0, so that the unrolled rescan of a full list
     // never picks one (every distance is at least 1e-7)
     for (int i = p.k; i < MAXK; i++) kdist[i * NT + threadIdx.x] = 0.0;
-    const int lane = threadIdx.x & 63;
-    const int wave = NT == 256 ? (int)(threadIdx.x >> 6) : (int)(blockIdx.x & 3);
+    const int lane = threadIdx.x;
+    const int wave = (int)(blockIdx.x & 3);  // the quarter of the tile
     // 16 x 16 tiles, launched in row-major tile order or in the order of a
     // load model (heaviest first)
-    const unsigned tslot = NT == 256 ? blockIdx.x : blockIdx.x >> 2;
+    const unsigned tslot = blockIdx.x >> 2;
     const int tiles_x = (p.cols + 15) >> 4;
     const int tile = p.order ? p.order[tslot] : (int)tslot;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
@@ -738,9 +736,8 @@ __global__ __launch_bounds__(idw_lanes<ADJOINT>()) void idw_kernel(IdwParams p)
 template <int MAXK, bool ADJOINT>
 static void launch_idw_k(const IdwParams &p, hipStream_t s)
 {
-    constexpr int nt = idw_lanes<ADJOINT>();
-    dim3 block(nt);
-    dim3 grid((unsigned)(((p.cols + 15) / 16) * ((p.rows + 15) / 16)) * (256 / nt));
+    dim3 block(kIdwLanes);
+    dim3 grid((unsigned)(((p.cols + 15) / 16) * ((p.rows + 15) / 16)) * 4u);
     hipLaunchKernelGGL((idw_kernel<MAXK, ADJOINT>), grid, block, 0, s, p);
 }
 
