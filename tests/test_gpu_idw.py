@@ -62,6 +62,14 @@ def test_buffer_rule(buffer):
         assert (got == 0).any()
 
 
+def test_many_samples_wide_slots():
+    # more than 65,536 bucket slots: the K-lists keep 32-bit slots (smaller
+    # sample sets take the kernel with 16-bit slots, as every other test)
+    xs, ys, vs = rand_samples(70000, 11, 3000.0)
+    compare(xs, ys, vs, 30.0, 12, 2, 0.0, 300.0)
+    compare(xs, ys, vs, 30.0, 8, 1, 0.0, 300.0)
+
+
 def test_synthetic_samples_small_grid():
     # the benchmark's sample recipe on a 512 x 512 grid (15,360 m square)
     xs, ys, vs = synth.samples(synth.BASE_SEED + 2, 2500, width=15360.0, height=15360.0)
