@@ -143,7 +143,36 @@ __device__ __forceinline__ T load_uniform(const T *ptr)
 
 // The reference's distance from its squared form (idwOperator.cpp:398-402):
 // the square root, or 1e-7 for a coincident sample.
-__device__ __forceinline__ double idw_dist(double s) { return s == 0.0 ? kIdwEps : sqrt(s); }
+//
+// The square root for s in [2^-767, 2^1000] is the hardware reciprocal
+// square root estimate refined by Goldschmidt's iteration and two Newton
+// corrections with fused multiply-adds (Markstein's method): the operations
+// the compiler issues for an IEEE double sqrt, without the scaling it adds
+// for tiny inputs and the selects for zero and infinity, which this range
+// excludes (the same result bits).  Other s take sqrt().
+__device__ __forceinline__ double idw_sqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    // a wave-uniform test first, and the argument made opaque inside it
+    // (an empty asm), so that the compiler cannot compute the full sqrt
+    // for every lane alongside and select
+    const bool in_range = x >= 0x1p-767 && x <= 0x1p+1000;
+    if (__builtin_amdgcn_ballot_w64(!in_range) != 0) {
+        double xo = x;
+        asm volatile("" : "+v"(xo));
+        if (!in_range) g = sqrt(xo);
+    }
+    return g;
+}
+__device__ __forceinline__ double idw_dist(double s) { return s == 0.0 ? kIdwEps : idw_sqrt(s); }
 
 // rowAtOtherLayer: (int)min(scale * idx, other - 1) with double scale.
 __device__ __forceinline__ int mask_index(int other, int self, int idx)
