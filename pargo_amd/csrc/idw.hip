@@ -487,9 +487,10 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
     int staged = 0;
     // visits the staged candidates bin by bin, nearest bins first; stops at
     // the first bin whose inner edge no lane's threshold reaches.  A
-    // candidate's bin b satisfies sqrt(lb2) >= b w (1 - 3e-7) (four float
-    // roundings), lb2 <= its squared distance as the test computes it
-    // (1 + 2^-51), so with the margins below no skipped candidate passes.
+    // candidate's bin b satisfies sqrt(lb2) >= b w (1 - 3e-7) (three float
+    // roundings and the hardware square root's 1 ulp: at most 2.25 ulp),
+    // lb2 <= its squared distance as the test computes it (1 + 2^-51), so
+    // with the margins below no skipped candidate passes.
     // positions and bins of the staged slots: every lane's loads are
     // requested before the first is used, one wait per flush rather than one
     // per block
@@ -503,7 +504,7 @@ __device__ __forceinline__ int idw_search_binned(const IdwParams &p, int blockRo
             if (64 * k >= staged) continue;
             const double ex = fmax(fmax(gx0 - xy[k].x, xy[k].x - gx1), 0.0);
             const double ey = fmax(fmax(gy0 - xy[k].y, xy[k].y - gy1), 0.0);
-            const float r = __builtin_sqrtf((float)(ex * ex + ey * ey)) * inv_w;
+            const float r = __builtin_amdgcn_sqrtf((float)(ex * ex + ey * ey)) * inv_w;
             sx[k] = xy[k].x;
             sy[k] = xy[k].y;
             sqb[k] = (sqb[k] << kBinBits) | (r < (float)(kBins - 1) ? (int)r : kBins - 1);  // NaN: last bin
